@@ -1,0 +1,116 @@
+/*
+ * licom_sw.h — C-ABI of the MI355X (gfx950) barotropic shallow-water step.
+ *
+ * All six cube faces of one resolution live on one GPU as (6, N, N) arrays,
+ * N = nx + 6 (halo 3), C-contiguous, j (last axis) fastest.  Device pointers
+ * are plain addresses of device memory; `stream` is a hipStream_t (or null
+ * for the default stream).  Every entry point returns 0 on success or a
+ * negative SW_E* code; sw_last_error() then describes the failure (the text
+ * is thread-local).  No entry point takes torch types.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   sw_rk2 / sw_rk3      momentum/barotr.py:41-61 `_barotr_rk2_core` and
+ *                        :64-87 `_barotr_rk3_core` (the RK substep loop,
+ *                        stages from momentum/barotr_pure.py:100-168)
+ *   sw_stage             momentum/barotr_pure.py:100-168 `_step_rk_logic`
+ *   sw_halo_scalar       kernel/cube.py:130-134 `ext_scalar`
+ *                        (= communication.py:387-480 `update_domain` with
+ *                        px = py = 1, then cube.py:40-127 `cube_rmp`)
+ *   sw_halo_vector       kernel/cube.py:137-153 `ext_vector`
+ *   sw_edge_avg          kernel/communication.py:485-570
+ *                        `boundary_communication`
+ *   sw_routing_table     kernel/communication.py:57-87, 277-284 (the
+ *                        `_routing` table built by `Communication.configure`)
+ */
+#ifndef LICOM_SW_H
+#define LICOM_SW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sw_ctx sw_ctx;
+
+enum {
+  SW_F64 = 0,  /* state and stencil arithmetic in binary64 (the drop-in) */
+  SW_F32 = 1   /* state and stencil arithmetic in binary32 */
+};
+
+enum {
+  SW_OK = 0,
+  SW_E_ARG = -1,     /* invalid argument (null pointer, bad size, bad dtype) */
+  SW_E_HIP = -2,     /* a HIP runtime call failed */
+  SW_E_CONST = -3,   /* consts violate an assumption (see sw_last_error) */
+  SW_E_STATE = -4    /* precondition on the state failed */
+};
+
+/* Flags for sw_rk2 / sw_rk3. */
+enum {
+  SW_SKIP_DIAG = 1   /* do not write the 10 diagnostic outputs */
+};
+
+/* Number of consts and their order: mymodule/schedule.py:95-116
+ *   0 dzph_x   1 dzph_y   2 rdx   3 rdy   4 a_f
+ *   5 k2e_coef (6,N,N,2)  6 loc_i_local (int32)  7 loc_j_local (int32)
+ *   8 a_c2l (6,N,N,2,2)   9 a_l2c (6,N,N,2,2)  10 inner  11 outer
+ *  12 a_gct (6,N,N,2,2)  13 a_sina  14 rda  15 c_dy  16 d_dx
+ * Float consts are binary64 device arrays; they are copied (and, for SW_F32,
+ * rounded) into the context, so the caller may free them after sw_create. */
+#define SW_N_CONSTS 17
+
+/* Create a context for resolution C{nx} (6 faces of (nx+6)^2) whose state is
+ * of `dtype`.  Validates the index tables (loc values in range) and the
+ * masks (inner = 1 on the interior and 0 on the halo, outer = 1 - inner). */
+int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CONSTS],
+              void* stream);
+void sw_destroy(sw_ctx* ctx);
+
+/* Bind the stream later launches go to. */
+int sw_set_stream(sw_ctx* ctx, void* stream);
+
+/* nbb RK2-midpoint substeps of length dtb (barotr.py:41-61).
+ *   in[6]  = h0, h0p, ub, vb, ubp, vbp          (read only; may alias)
+ *   out[13]= h0, ub, vb, celerity_x, celerity_y, ub_ct, vb_ct, ub_cx, ub_cy,
+ *            vb_cx, vb_cy, advx, advy          (distinct from in[])
+ * The reference returns h0p = h0, ubp = ub, vbp = vb and passes the coupling
+ * fields through; callers alias those.  nbb = 0 copies in -> out (h0, ub, vb)
+ * and leaves the diagnostics untouched, as the reference's empty loop does. */
+int sw_rk2(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
+           double dtb, int flags);
+/* nbb RK3 substeps (barotr.py:64-87): stages dt/3, dt/2, dt with beta = 1. */
+int sw_rk3(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
+           double dtb, int flags);
+
+/* One RK stage (barotr_pure.py:100-168); out[13] as sw_rk2, in the order of
+ * the reference's 13-tuple.  beta is the forward-backward blend weight. */
+int sw_stage(sw_ctx* ctx, const void* const in[6], void* const out[13], double dt,
+             double beta, int is_last);
+
+/* In-place duo-grid halo fill of `ncomp` stacked fields laid out
+ * (6, ncomp, N, N) (ncomp = 1 for (6, N, N)). */
+int sw_halo_scalar(sw_ctx* ctx, void* field, int ncomp);
+/* In-place vector halo fill of covariant (u, v) via lat-lon rotation. */
+int sw_halo_vector(sw_ctx* ctx, void* u, void* v);
+/* In-place C-grid edge averaging: u holds the axis -2 normal component on the
+ * N/S edges, v the axis -1 normal component on the E/W edges. */
+int sw_edge_avg(sw_ctx* ctx, void* u, void* v);
+
+/* Wait for all work the context launched. */
+int sw_sync(sw_ctx* ctx);
+
+/* Host-only: the cube routing table R[t][d] = (neighbour face, its edge,
+ * transform 0=ID / 1=FLIP), directions E=0, W=1, N=2, S=3. */
+int sw_routing_table(int32_t out[6 * 4 * 3]);
+
+/* Host-only: resolution and dtype of a context. */
+int sw_info(const sw_ctx* ctx, int* nx, int* dtype);
+
+const char* sw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LICOM_SW_H */

@@ -1,0 +1,1309 @@
+// MI355X (gfx950) kernels for the barotropic RK step on the cubed-sphere
+// duo-grid, behind the C-ABI declared in include/licom_sw.h.
+//
+// Layout: every field is (6, N, N), N = nx + 6, C-contiguous (j fastest);
+// halo width 3; interior i, j in [3, N-3).  All six faces live on one GPU, so
+// the reference's cross-face exchanges (all_gather + routing transforms,
+// kernel/communication.py:355-570) become plain loads from the neighbouring
+// face's memory.
+//
+// One RK stage (momentum/barotr_pure.py:100-168) runs as five launches:
+//   k_edge     C-grid normal velocity on every face edge, averaged with the
+//              neighbour's (remap.py:107 -> communication.py:485-564); in the
+//              last stage also the averaged mass flux (barotr_pure.py:116-117).
+//              Each edge line is computed by both faces from the same two
+//              operands, so both sides agree bit for bit.
+//   k_stage_a  new height on the interior: celerity, C-grid velocity, LMARS
+//              correction, upwind flux, divergence (barotr_pure.py:111-120).
+//   k_halo_h   duo-grid halo fill of the new height (cube.py:130-134).
+//   k_stage_b  new covariant velocity on the interior: D-grid upwind,
+//              vorticity/kinetic-energy advection, LMARS dissipation, pressure
+//              gradient, Coriolis (barotr_pure.py:133-150).
+//   k_halo_uv  vector halo fill (cube.py:137-153).
+// The ten diagnostic outputs of the last stage are written once per call by
+// k_diag over the full padded arrays, with the reference's values on the
+// halo ring.  Stage kernels stage their stencil footprints (height, velocity
+// components, contravariant velocity) in LDS tiles; every multiply and add is
+// evaluated in the reference's order and the library is compiled with
+// -ffp-contract=off, so the binary64 path reproduces the NumPy restatement of
+// the reference exactly.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "licom_sw.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define SW_HIP(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(SW_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// topology: directions E=0, W=1, N=2, S=3 (communication.py:32)
+// ---------------------------------------------------------------------------
+constexpr int DE = 0, DW = 1, DN = 2, DS = 3;
+constexpr int HALO = 3;
+
+struct Route {
+  int r[6][4][3];  // (neighbour face, neighbour edge, 1 if the joint flips)
+};
+
+// Face joints of the cube: even faces attach S->(t+4):E, E->(t+2):S,
+// N->(t+1):S, W->(t+5):E; odd faces E->(t+1):W, N->(t+2):W, W->(t+4):N,
+// S->(t+5):N (communication.py:57-87).  A joint between parallel edges keeps
+// the along-edge order; a joint between perpendicular edges reverses it
+// (communication.py:40-55).
+Route build_route() {
+  Route R;
+  std::memset(&R, 0, sizeof(R));
+  for (int t = 0; t < 6; ++t) {
+    int nb[4], ed[4];
+    if (t % 2 == 0) {
+      nb[DS] = (t + 4) % 6; ed[DS] = DE;
+      nb[DE] = (t + 2) % 6; ed[DE] = DS;
+      nb[DN] = (t + 1) % 6; ed[DN] = DS;
+      nb[DW] = (t + 5) % 6; ed[DW] = DE;
+    } else {
+      nb[DE] = (t + 1) % 6; ed[DE] = DW;
+      nb[DN] = (t + 2) % 6; ed[DN] = DW;
+      nb[DW] = (t + 4) % 6; ed[DW] = DN;
+      nb[DS] = (t + 5) % 6; ed[DS] = DN;
+    }
+    for (int d = 0; d < 4; ++d) {
+      const bool a_ew = (d == DE || d == DW);
+      const bool b_ew = (ed[d] == DE || ed[d] == DW);
+      const int flip = (a_ew != b_ew) ? 1 : 0;
+      // both directions of a joint are recorded, as the reference does
+      R.r[t][d][0] = nb[d];
+      R.r[t][d][1] = ed[d];
+      R.r[t][d][2] = flip;
+      R.r[nb[d]][ed[d]][0] = t;
+      R.r[nb[d]][ed[d]][1] = d;
+      R.r[nb[d]][ed[d]][2] = flip;
+    }
+  }
+  return R;
+}
+
+const Route& route() {
+  static const Route R = build_route();
+  return R;
+}
+
+// ---------------------------------------------------------------------------
+// stencil coefficients (operators/poly.py:29-37)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PC {
+  static constexpr T EP23 = T(-1.0 / 20.0);
+  static constexpr T EP13 = T(9.0 / 20.0);
+  static constexpr T EP03 = T(47.0 / 60.0);
+  static constexpr T EM13 = T(-13.0 / 60.0);
+  static constexpr T EM23 = T(1.0 / 30.0);
+  static constexpr T M13 = T(37.0 / 60.0);
+  static constexpr T M23 = T(-2.0 / 15.0);
+  static constexpr T M33 = T(1.0 / 60.0);
+};
+
+// East/north 5-point face value from samples at k-2..k+2 (poly.py:50-56).
+template <typename T>
+__device__ __forceinline__ T e5(T a, T b, T c, T d, T e) {
+  return PC<T>::EM23 * a + PC<T>::EM13 * b + PC<T>::EP03 * c + PC<T>::EP13 * d + PC<T>::EP23 * e;
+}
+// West/south 5-point face value (poly.py:57-63).
+template <typename T>
+__device__ __forceinline__ T w5(T a, T b, T c, T d, T e) {
+  return PC<T>::EP23 * a + PC<T>::EP13 * b + PC<T>::EP03 * c + PC<T>::EM13 * d + PC<T>::EM23 * e;
+}
+// 6-point centred value from samples at k-3..k+2 (poly.py:120-124).
+template <typename T>
+__device__ __forceinline__ T c6(T m3, T m2, T m1, T p0, T p1, T p2) {
+  return PC<T>::M13 * (m1 + p0) + PC<T>::M23 * (m2 + p1) + PC<T>::M33 * (m3 + p2);
+}
+// jnp.maximum(x, 10.0): NaN propagates.
+template <typename T>
+__device__ __forceinline__ T max10(T x) {
+  return (x >= T(10.0) || x != x) ? x : T(10.0);
+}
+template <typename T>
+__device__ __forceinline__ T celerity(T centred, T dz) {
+  return max10<T>(sqrt(T(9.8) * (centred + dz)));
+}
+
+// ---------------------------------------------------------------------------
+// context-owned const planes (each (6, N, N))
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Planes {
+  const T *dzx, *dzy, *rdx, *rdy, *af, *rda, *cdy, *ddx, *sina;
+  const T *g00, *g01, *g10, *g11;
+  const T *c00, *c01, *c10, *c11;
+  const T *l00, *l01, *l10, *l11;
+  const T *k0, *k1;
+  const int *li, *lj;
+  T dzx_s, dzy_s;
+  int dz_uniform;
+};
+
+struct Geo {
+  int nx, N;
+  __host__ __device__ size_t at(int t, int i, int j) const {
+    return (static_cast<size_t>(t) * N + i) * N + j;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T dzx(const Planes<T>& P, size_t o) {
+  return P.dz_uniform ? P.dzx_s : P.dzx[o];
+}
+template <typename T>
+__device__ __forceinline__ T dzy(const Planes<T>& P, size_t o) {
+  return P.dz_uniform ? P.dzy_s : P.dzy[o];
+}
+template <typename T>
+__device__ __forceinline__ T uct_at(const Planes<T>& P, const T* u, const T* v, size_t o) {
+  return (P.g00[o] * u[o] + P.g01[o] * v[o]) * P.sina[o];
+}
+template <typename T>
+__device__ __forceinline__ T vct_at(const Planes<T>& P, const T* u, const T* v, size_t o) {
+  return (P.g10[o] * u[o] + P.g11[o] * v[o]) * P.sina[o];
+}
+
+// ---------------------------------------------------------------------------
+// edge lines
+// ---------------------------------------------------------------------------
+// C-grid normal velocity at position r of edge d of face t, before averaging
+// (remap.py:42-55 at the edge rows/columns 3 and N-3).
+template <typename T>
+__device__ T edge_normal_velocity(const Geo& g, const Planes<T>& P, const T* u, const T* v,
+                                  int t, int d, int r) {
+  T w[6];
+  if (d == DN || d == DS) {
+    const int ie = (d == DS) ? 3 : g.N - 3;
+    const int j = 3 + r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = uct_at(P, u, v, g.at(t, ie - 3 + k, j));
+  } else {
+    const int je = (d == DW) ? 3 : g.N - 3;
+    const int i = 3 + r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = vct_at(P, u, v, g.at(t, i, je - 3 + k));
+  }
+  return T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
+}
+
+// Upwind mass flux through edge d of face t at position r for the averaged
+// normal velocity vavg (barotr_pure.py:29-49 at the edge line).
+template <typename T>
+__device__ T edge_flux(const Geo& g, const Planes<T>& P, const T* h, int t, int d, int r, T vavg) {
+  T s[6];
+  size_t oc;
+  T dz;
+  if (d == DN || d == DS) {
+    const int ie = (d == DS) ? 3 : g.N - 3;
+    const int j = 3 + r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = h[g.at(t, ie - 3 + k, j)];
+    oc = g.at(t, ie, j);
+    dz = dzx(P, oc);
+  } else {
+    const int je = (d == DW) ? 3 : g.N - 3;
+    const int i = 3 + r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = h[g.at(t, i, je - 3 + k)];
+    oc = g.at(t, i, je);
+    dz = dzy(P, oc);
+  }
+  const T he = e5(s[0], s[1], s[2], s[3], s[4]);
+  const T hw = w5(s[1], s[2], s[3], s[4], s[5]);
+  const T c = celerity(c6(s[0], s[1], s[2], s[3], s[4], s[5]), dz);
+  const T vc = vavg + T(4.9) / c * (he - hw);
+  return dz * vc + (vc > T(0) ? he : hw) * vc;
+}
+
+template <typename T>
+struct EdgeArgs {
+  Geo g;
+  Route R;
+  Planes<T> P;
+  const T *h, *u, *v;
+  T *evel, *eflux;
+  int last;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge(EdgeArgs<T> a) {
+  const int nx = a.g.nx;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 24 * nx) return;
+  const int t = q / (4 * nx);
+  const int d = (q / nx) % 4;
+  const int r = q % nx;
+  const int tb = a.R.r[t][d][0], db = a.R.r[t][d][1];
+  const int rr = a.R.r[t][d][2] ? nx - 1 - r : r;
+  const T mine = edge_normal_velocity(a.g, a.P, a.u, a.v, t, d, r);
+  const T nb = edge_normal_velocity(a.g, a.P, a.u, a.v, tb, db, rr);
+  const T avg = (mine + nb) / T(2.0);
+  a.evel[q] = avg;
+  if (a.last) {
+    const T fm = edge_flux(a.g, a.P, a.h, t, d, r, avg);
+    const T fn = edge_flux(a.g, a.P, a.h, tb, db, rr, avg);
+    a.eflux[q] = (fm + fn) / T(2.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// stage kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+struct StageArgs {
+  Geo g;
+  Planes<T> P;
+  const T *h, *hp, *u, *v, *up, *vp;  // stage inputs
+  const T *hn;                          // new height with filled halo (stage B)
+  const T *evel, *eflux;
+  T *h_out, *u_out, *v_out;
+  T dt, beta;
+  int last;
+  int nti, ntj;
+};
+
+constexpr int kThreads = 256;
+
+template <int TI, int TJ>
+__device__ __forceinline__ void tile_origin(int nti, int ntj, int& t, int& i0, int& j0) {
+  const int per_face = nti * ntj;
+  const int b = blockIdx.x;
+  t = b / per_face;
+  const int rem = b - t * per_face;
+  const int ti = rem / ntj;
+  const int tj = rem - ti * ntj;
+  i0 = HALO + ti * TI;
+  j0 = HALO + tj * TJ;
+}
+
+// New height on the interior of one TI x TJ tile.
+template <typename T, int TI, int TJ>
+__global__ __launch_bounds__(kThreads) void k_stage_a(StageArgs<T> a) {
+  constexpr int HR = TI + 6, HC = TJ + 6, FJ = TJ + 1;
+  __shared__ T s_h[HR * HC];
+  __shared__ T s_uct[HR * TJ];
+  __shared__ T s_vct[TI * HC];
+  __shared__ T s_F[(TI + 1) * TJ];
+  __shared__ T s_G[TI * FJ];
+
+  const Geo g = a.g;
+  const int N = g.N, nx = g.nx;
+  int t, i0, j0;
+  tile_origin<TI, TJ>(a.nti, a.ntj, t, i0, j0);
+  const int ni = min(TI, N - HALO - i0), nj = min(TJ, N - HALO - j0);
+  const Planes<T>& P = a.P;
+
+  for (int q = threadIdx.x; q < HR * HC; q += kThreads) {
+    const int r = q / HC, c = q - r * HC;
+    if (r < ni + 6 && c < nj + 6) s_h[q] = a.h[g.at(t, i0 - 3 + r, j0 - 3 + c)];
+  }
+  for (int q = threadIdx.x; q < HR * TJ; q += kThreads) {
+    const int r = q / TJ, c = q - r * TJ;
+    if (r < ni + 6 && c < nj) s_uct[q] = uct_at(P, a.u, a.v, g.at(t, i0 - 3 + r, j0 + c));
+  }
+  for (int q = threadIdx.x; q < TI * HC; q += kThreads) {
+    const int r = q / HC, c = q - r * HC;
+    if (r < ni && c < nj + 6) s_vct[q] = vct_at(P, a.u, a.v, g.at(t, i0 + r, j0 - 3 + c));
+  }
+  __syncthreads();
+
+  // mass flux through the i-faces (rows i0 .. i0+ni)
+  for (int q = threadIdx.x; q < (TI + 1) * TJ; q += kThreads) {
+    const int k = q / TJ, c = q - k * TJ;
+    if (k > ni || c >= nj) continue;
+    const int i = i0 + k, j = j0 + c;
+    const bool edge = (i == HALO) || (i == N - HALO);
+    const int eidx = (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO);
+    T F;
+    if (a.last && edge) {
+      F = a.eflux[eidx];
+    } else {
+      const T* hc = &s_h[k * HC + c + 3];
+      const T he = e5(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC]);
+      const T hw = w5(hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]);
+      T upre;
+      if (edge) {
+        upre = a.evel[eidx];
+      } else {
+        const T* uc = &s_uct[k * TJ + c];
+        upre = T(0.5) * (e5(uc[0], uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ]) +
+                         w5(uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ], uc[5 * TJ]));
+      }
+      const size_t o = g.at(t, i, j);
+      const T dz = dzx(P, o);
+      const T cx = celerity(c6(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]), dz);
+      const T uc = upre + T(4.9) / cx * (he - hw);
+      F = dz * uc + (uc > T(0) ? he : hw) * uc;
+    }
+    s_F[q] = F;
+  }
+  // mass flux through the j-faces (columns j0 .. j0+nj)
+  for (int q = threadIdx.x; q < TI * FJ; q += kThreads) {
+    const int k = q / FJ, c = q - k * FJ;
+    if (k >= ni || c > nj) continue;
+    const int i = i0 + k, j = j0 + c;
+    const bool edge = (j == HALO) || (j == N - HALO);
+    const int eidx = (t * 4 + (j == HALO ? DW : DE)) * nx + (i - HALO);
+    T G;
+    if (a.last && edge) {
+      G = a.eflux[eidx];
+    } else {
+      const T* hr = &s_h[(k + 3) * HC + c];
+      const T hn = e5(hr[0], hr[1], hr[2], hr[3], hr[4]);
+      const T hs = w5(hr[1], hr[2], hr[3], hr[4], hr[5]);
+      T vpre;
+      if (edge) {
+        vpre = a.evel[eidx];
+      } else {
+        const T* vc = &s_vct[k * HC + c];
+        vpre = T(0.5) * (e5(vc[0], vc[1], vc[2], vc[3], vc[4]) + w5(vc[1], vc[2], vc[3], vc[4], vc[5]));
+      }
+      const size_t o = g.at(t, i, j);
+      const T dz = dzy(P, o);
+      const T cy = celerity(c6(hr[0], hr[1], hr[2], hr[3], hr[4], hr[5]), dz);
+      const T vc = vpre + T(4.9) / cy * (hn - hs);
+      G = dz * vc + (vc > T(0) ? hn : hs) * vc;
+    }
+    s_G[q] = G;
+  }
+  __syncthreads();
+
+  // divergence and height update (agrid.py:44-53, barotr_pure.py:120)
+  for (int q = threadIdx.x; q < TI * TJ; q += kThreads) {
+    const int k = q / TJ, c = q - k * TJ;
+    if (k >= ni || c >= nj) continue;
+    const size_t o = g.at(t, i0 + k, j0 + c);
+    const T D = P.rda[o] * (((s_F[(k + 1) * TJ + c] * P.cdy[o + N] - s_F[k * TJ + c] * P.cdy[o]) +
+                             s_G[k * FJ + c + 1] * P.ddx[o + 1]) -
+                            s_G[k * FJ + c] * P.ddx[o]);
+    a.h_out[o] = a.hp[o] - D * a.dt;
+  }
+}
+
+// New covariant velocity on the interior of one TI x TJ tile.
+template <typename T, int TI, int TJ>
+__global__ __launch_bounds__(kThreads) void k_stage_b(StageArgs<T> a) {
+  constexpr int HR = TI + 6, HC = TJ + 6, FJ = TJ + 1;
+  __shared__ T s_h[HR * HC];    // stage-input height (rect; corners unused)
+  __shared__ T s_n[HR * HC];    // blended new height (1-beta) hn + beta h
+  __shared__ T s_uctV[HR * TJ]; // contravariant u along i
+  __shared__ T s_vV[HR * TJ];   // covariant v along i
+  __shared__ T s_vctH[TI * HC]; // contravariant v along j
+  __shared__ T s_uH[TI * HC];   // covariant u along j
+  __shared__ T s_ucx[(TI + 1) * TJ], s_vcx[(TI + 1) * TJ], s_Cx[(TI + 1) * TJ];
+  __shared__ T s_vcy[TI * FJ], s_ucy[TI * FJ], s_Cy[TI * FJ];
+
+  const Geo g = a.g;
+  const int N = g.N, nx = g.nx;
+  int t, i0, j0;
+  tile_origin<TI, TJ>(a.nti, a.ntj, t, i0, j0);
+  const int ni = min(TI, N - HALO - i0), nj = min(TJ, N - HALO - j0);
+  const Planes<T>& P = a.P;
+  const T wn = T(1.0) - a.beta, wo = a.beta;
+
+  for (int q = threadIdx.x; q < HR * HC; q += kThreads) {
+    const int r = q / HC, c = q - r * HC;
+    if (r < ni + 6 && c < nj + 6) {
+      const size_t o = g.at(t, i0 - 3 + r, j0 - 3 + c);
+      const T ho = a.h[o];
+      s_h[q] = ho;
+      s_n[q] = wn * a.hn[o] + wo * ho;
+    }
+  }
+  for (int q = threadIdx.x; q < HR * TJ; q += kThreads) {
+    const int r = q / TJ, c = q - r * TJ;
+    if (r < ni + 6 && c < nj) {
+      const size_t o = g.at(t, i0 - 3 + r, j0 + c);
+      s_uctV[q] = uct_at(P, a.u, a.v, o);
+      s_vV[q] = a.v[o];
+    }
+  }
+  for (int q = threadIdx.x; q < TI * HC; q += kThreads) {
+    const int r = q / HC, c = q - r * HC;
+    if (r < ni && c < nj + 6) {
+      const size_t o = g.at(t, i0 + r, j0 - 3 + c);
+      s_vctH[q] = vct_at(P, a.u, a.v, o);
+      s_uH[q] = a.u[o];
+    }
+  }
+  __syncthreads();
+
+  // C-grid quantities on the i-faces
+  for (int q = threadIdx.x; q < (TI + 1) * TJ; q += kThreads) {
+    const int k = q / TJ, c = q - k * TJ;
+    if (k > ni || c >= nj) continue;
+    const int i = i0 + k, j = j0 + c;
+    const bool edge = (i == HALO) || (i == N - HALO);
+    const T* uc = &s_uctV[k * TJ + c];
+    const T ue = e5(uc[0], uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ]);
+    const T uw = w5(uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ], uc[5 * TJ]);
+    const T upre = edge ? a.evel[(t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO)] : T(0.5) * (ue + uw);
+    const T* vv = &s_vV[k * TJ + c];
+    const T vcx = upre > T(0) ? e5(vv[0], vv[TJ], vv[2 * TJ], vv[3 * TJ], vv[4 * TJ])
+                              : w5(vv[TJ], vv[2 * TJ], vv[3 * TJ], vv[4 * TJ], vv[5 * TJ]);
+    const T* hc = &s_h[k * HC + c + 3];
+    const T he = e5(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC]);
+    const T hw = w5(hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]);
+    const size_t o = g.at(t, i, j);
+    const T cx = celerity(c6(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]), dzx(P, o));
+    s_ucx[q] = upre + T(4.9) / cx * (he - hw);
+    s_vcx[q] = vcx;
+    s_Cx[q] = T(0.5) * cx * (ue - uw);
+  }
+  // C-grid quantities on the j-faces
+  for (int q = threadIdx.x; q < TI * FJ; q += kThreads) {
+    const int k = q / FJ, c = q - k * FJ;
+    if (k >= ni || c > nj) continue;
+    const int i = i0 + k, j = j0 + c;
+    const bool edge = (j == HALO) || (j == N - HALO);
+    const T* vc = &s_vctH[k * HC + c];
+    const T vn = e5(vc[0], vc[1], vc[2], vc[3], vc[4]);
+    const T vs = w5(vc[1], vc[2], vc[3], vc[4], vc[5]);
+    const T vpre = edge ? a.evel[(t * 4 + (j == HALO ? DW : DE)) * nx + (i - HALO)] : T(0.5) * (vn + vs);
+    const T* uu = &s_uH[k * HC + c];
+    const T ucy = vpre > T(0) ? e5(uu[0], uu[1], uu[2], uu[3], uu[4]) : w5(uu[1], uu[2], uu[3], uu[4], uu[5]);
+    const T* hr = &s_h[(k + 3) * HC + c];
+    const T hn = e5(hr[0], hr[1], hr[2], hr[3], hr[4]);
+    const T hs = w5(hr[1], hr[2], hr[3], hr[4], hr[5]);
+    const size_t o = g.at(t, i, j);
+    const T cy = celerity(c6(hr[0], hr[1], hr[2], hr[3], hr[4], hr[5]), dzy(P, o));
+    s_vcy[q] = vpre + T(4.9) / cy * (hn - hs);
+    s_ucy[q] = ucy;
+    s_Cy[q] = T(0.5) * cy * (vn - vs);
+  }
+  __syncthreads();
+
+  for (int q = threadIdx.x; q < TI * TJ; q += kThreads) {
+    const int k = q / TJ, c = q - k * TJ;
+    if (k >= ni || c >= nj) continue;
+    const int i = i0 + k, j = j0 + c;
+    const size_t o = g.at(t, i, j);
+    const int x0 = k * TJ + c, x1 = (k + 1) * TJ + c;  // i-face rows k, k+1
+    const int y0 = k * FJ + c, y1 = k * FJ + c + 1;    // j-face cols c, c+1
+    const T rdx = P.rdx[o], rdy = P.rdy[o];
+    // vorticity (agrid.py:32-41) with vv = vb_cx, uu = ub_cy
+    const T zeta = P.rda[o] * (((s_ucy[y0] * P.ddx[o] + s_vcx[x1] * P.cdy[o + N]) - s_ucy[y1] * P.ddx[o + 1]) -
+                               s_vcx[x0] * P.cdy[o]);
+    const T ku1 = T(0.5) * (s_ucx[x1] * s_ucx[x1] + s_vcx[x1] * s_vcx[x1]);
+    const T ku0 = T(0.5) * (s_ucx[x0] * s_ucx[x0] + s_vcx[x0] * s_vcx[x0]);
+    const T kv1 = T(0.5) * (s_vcy[y1] * s_vcy[y1] + s_ucy[y1] * s_ucy[y1]);
+    const T kv0 = T(0.5) * (s_vcy[y0] * s_vcy[y0] + s_ucy[y0] * s_ucy[y0]);
+    const T vct = s_vctH[k * HC + c + 3];
+    const T uct = s_uctV[(k + 3) * TJ + c];
+    const T advx = zeta * vct - (ku1 - ku0) * rdx;
+    const T advy = -zeta * uct - (kv1 - kv0) * rdy;
+    const T visx = -(rdx * (s_Cx[x1] - s_Cx[x0]));
+    const T visy = -(rdy * (s_Cy[y1] - s_Cy[y0]));
+    const T* nc = &s_n[k * HC + c + 3];  // column j, rows i-3 .. i+3
+    const T hx0 = c6(nc[0], nc[HC], nc[2 * HC], nc[3 * HC], nc[4 * HC], nc[5 * HC]);
+    const T hx1 = c6(nc[HC], nc[2 * HC], nc[3 * HC], nc[4 * HC], nc[5 * HC], nc[6 * HC]);
+    const T* nr = &s_n[(k + 3) * HC + c];  // row i, cols j-3 .. j+3
+    const T hy0 = c6(nr[0], nr[1], nr[2], nr[3], nr[4], nr[5]);
+    const T hy1 = c6(nr[1], nr[2], nr[3], nr[4], nr[5], nr[6]);
+    const T pgfu = T(-9.80) * (rdx * (hx1 - hx0));
+    const T pgfv = T(-9.80) * (rdy * (hy1 - hy0));
+    const T af = P.af[o];
+    a.u_out[o] = a.up[o] + (((pgfu + advx) + af * vct) + visx) * a.dt;
+    a.v_out[o] = a.vp[o] + (((pgfv + advy) - af * uct) + visy) * a.dt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// duo-grid halo fill (update_domain pass 1 + cube_rmp, closed index form)
+// ---------------------------------------------------------------------------
+// Interior cell at distance layer (1..3) inside edge d, along-edge index r.
+__device__ __forceinline__ void interior_cell(int N, int d, int layer, int r, int& i, int& j) {
+  if (d == DE) { i = 3 + r; j = N - 3 - layer; }
+  else if (d == DW) { i = 3 + r; j = 2 + layer; }
+  else if (d == DN) { i = N - 3 - layer; j = 3 + r; }
+  else { i = 2 + layer; j = 3 + r; }
+}
+
+// Source offset of the value copied into the halo of face t, edge d, layer,
+// at along-edge index r (communication.py:404-430).
+__device__ __forceinline__ size_t copy_source(const Geo& g, const Route& R, int t, int d, int layer, int r) {
+  const int tb = R.r[t][d][0], db = R.r[t][d][1];
+  const int rr = R.r[t][d][2] ? g.nx - 1 - r : r;
+  int i, j;
+  interior_cell(g.N, db, layer, rr, i, j);
+  return g.at(tb, i, j);
+}
+
+// Strip cell (i, j) of face t: its edge, layer and the index along the strip.
+// Returns the two interpolation sources (cube.py:64-94).
+template <typename T>
+__device__ __forceinline__ void strip_sources(const Geo& g, const Route& R, const Planes<T>& P, int t,
+                                              int i, int j, size_t& s0, size_t& s1, T& c0, T& c1) {
+  const int N = g.N;
+  const size_t o = g.at(t, i, j);
+  c0 = P.k0[o];
+  c1 = P.k1[o];
+  if (j < 3 || j >= N - 3) {            // W / E strips interpolate along i
+    const int d = j < 3 ? DW : DE;
+    const int layer = j < 3 ? 3 - j : j - (N - 4);
+    const int L = P.li[o];
+    s0 = copy_source(g, R, t, d, layer, L - 3);
+    s1 = copy_source(g, R, t, d, layer, L + 1 - 3);
+  } else {                              // S / N strips interpolate along j
+    const int d = i < 3 ? DS : DN;
+    const int layer = i < 3 ? 3 - i : i - (N - 4);
+    const int L = P.lj[o];
+    s0 = copy_source(g, R, t, d, layer, L - 3);
+    s1 = copy_source(g, R, t, d, layer, L + 1 - 3);
+  }
+}
+
+// Halo cell q of a face (strips then corner blocks) -> (i, j) of the cell and
+// (si, sj) of the strip cell whose remapped value it takes (itself for strip
+// cells; corners copy row 3 / N-4 of the W/E strips, cube.py:96-125).
+__device__ __forceinline__ void halo_cell(int N, int nx, int q, int& i, int& j, int& si, int& sj) {
+  const int strip = 12 * nx;
+  if (q < strip) {
+    const int d = q / (3 * nx);
+    const int rem = q - d * 3 * nx;
+    const int layer = 1 + rem / nx;
+    const int r = rem % nx;
+    if (d == DE) { i = 3 + r; j = N - 4 + layer; }
+    else if (d == DW) { i = 3 + r; j = 3 - layer; }
+    else if (d == DN) { i = N - 4 + layer; j = 3 + r; }
+    else { i = 3 - layer; j = 3 + r; }
+    si = i;
+    sj = j;
+  } else {
+    const int c = q - strip;          // 0..35
+    const int corner = c / 9, w = c % 9;
+    const int a = w / 3, b = w % 3;
+    const bool north = corner >= 2, east = (corner & 1) != 0;
+    i = north ? N - 3 + a : a;
+    j = east ? N - 3 + b : b;
+    si = north ? N - 4 : 3;
+    sj = j;
+  }
+}
+
+template <typename T>
+struct HaloArgs {
+  Geo g;
+  Route R;
+  Planes<T> P;
+  T *f;        // scalar field base (6, ncomp, N, N)
+  int ncomp;
+  T *u, *v;    // vector pair
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_scalar(HaloArgs<T> a) {
+  const Geo g = a.g;
+  const int per_face = 12 * g.nx + 36;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 6 * per_face) return;
+  const int t = q / per_face;
+  int i, j, si, sj;
+  halo_cell(g.N, g.nx, q - t * per_face, i, j, si, sj);
+  size_t s0, s1;
+  T c0, c1;
+  strip_sources(g, a.R, a.P, t, si, sj, s0, s1, c0, c1);
+  const size_t NN = static_cast<size_t>(g.N) * g.N;
+  for (int comp = 0; comp < a.ncomp; ++comp) {
+    // face-major stacking: face f, component comp at (f * ncomp + comp) * NN
+    auto off = [&](size_t o) {
+      const size_t f = o / NN;
+      return (f * a.ncomp + comp) * NN + (o - f * NN);
+    };
+    const T val = c0 * a.f[off(s0)] + c1 * a.f[off(s1)];
+    a.f[off(g.at(t, i, j))] = val;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
+  const Geo g = a.g;
+  const int per_face = 12 * g.nx + 36;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 6 * per_face) return;
+  const int t = q / per_face;
+  int i, j, si, sj;
+  halo_cell(g.N, g.nx, q - t * per_face, i, j, si, sj);
+  size_t s0, s1;
+  T c0, c1;
+  strip_sources(g, a.R, a.P, t, si, sj, s0, s1, c0, c1);
+  const Planes<T>& P = a.P;
+  // rotate the copied interior values to lat-lon (cube.py:139-140; inner = 1)
+  const T ul0 = P.c00[s0] * a.u[s0] + P.c01[s0] * a.v[s0];
+  const T vl0 = P.c10[s0] * a.u[s0] + P.c11[s0] * a.v[s0];
+  const T ul1 = P.c00[s1] * a.u[s1] + P.c01[s1] * a.v[s1];
+  const T vl1 = P.c10[s1] * a.u[s1] + P.c11[s1] * a.v[s1];
+  const T p0 = c0 * ul0 + c1 * ul1;
+  const T p1 = c0 * vl0 + c1 * vl1;
+  // back to covariant with this cell's own rotation (cube.py:147-151; outer = 1)
+  const size_t o = g.at(t, i, j);
+  a.u[o] = P.l00[o] * p0 + P.l01[o] * p1;
+  a.v[o] = P.l10[o] * p0 + P.l11[o] * p1;
+}
+
+// ---------------------------------------------------------------------------
+// standalone edge averaging (communication.py:485-564), two phases
+// ---------------------------------------------------------------------------
+template <typename T>
+struct EdgeAvgArgs {
+  Geo g;
+  Route R;
+  T *u, *v, *buf;
+};
+
+__device__ __forceinline__ void edge_line_cell(int N, int d, int r, int& i, int& j) {
+  if (d == DE) { i = 3 + r; j = N - 3; }
+  else if (d == DW) { i = 3 + r; j = 3; }
+  else if (d == DN) { i = N - 3; j = 3 + r; }
+  else { i = 3; j = 3 + r; }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_avg_read(EdgeAvgArgs<T> a) {
+  const Geo g = a.g;
+  const int nx = g.nx;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 24 * nx) return;
+  const int t = q / (4 * nx), d = (q / nx) % 4, r = q % nx;
+  const int tb = a.R.r[t][d][0], db = a.R.r[t][d][1];
+  const int rr = a.R.r[t][d][2] ? nx - 1 - r : r;
+  int i, j, bi, bj;
+  edge_line_cell(g.N, d, r, i, j);
+  edge_line_cell(g.N, db, rr, bi, bj);
+  const T mine = (d == DE || d == DW) ? a.v[g.at(t, i, j)] : a.u[g.at(t, i, j)];
+  const T nb = (db == DE || db == DW) ? a.v[g.at(tb, bi, bj)] : a.u[g.at(tb, bi, bj)];
+  a.buf[q] = (mine + nb) / T(2.0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_avg_write(EdgeAvgArgs<T> a) {
+  const Geo g = a.g;
+  const int nx = g.nx;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 24 * nx) return;
+  const int t = q / (4 * nx), d = (q / nx) % 4, r = q % nx;
+  int i, j;
+  edge_line_cell(g.N, d, r, i, j);
+  if (d == DE || d == DW) a.v[g.at(t, i, j)] = a.buf[q];
+  else a.u[g.at(t, i, j)] = a.buf[q];
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics over the full padded arrays (the reference's returned fields)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DiagArgs {
+  Geo g;
+  Planes<T> P;
+  const T *h, *u, *v, *evel;
+  T *cx, *cy, *uct, *vct, *ucx, *ucy, *vcx, *vcy, *advx, *advy;
+};
+
+template <typename T>
+struct DiagCell {
+  const Geo& g;
+  const Planes<T>& P;
+  const T *h, *u, *v, *evel;
+  int t;
+
+  __device__ T U(int i, int j) const { return uct_at(P, u, v, g.at(t, i, j)); }
+  __device__ T V(int i, int j) const { return vct_at(P, u, v, g.at(t, i, j)); }
+  __device__ T Hh(int i, int j) const { return h[g.at(t, i, j)]; }
+
+  __device__ T cx(int i, int j) const {
+    const int N = g.N;
+    T s = T(0);
+    if (i >= 3 && i < N - 2 && j >= 3 && j < N - 2)
+      s = c6(Hh(i - 3, j), Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j), Hh(i + 2, j));
+    return celerity(s, dzx(P, g.at(t, i, j)));
+  }
+  __device__ T cy(int i, int j) const {
+    const int N = g.N;
+    T s = T(0);
+    if (i >= 3 && i < N - 2 && j >= 3 && j < N - 2)
+      s = c6(Hh(i, j - 3), Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1), Hh(i, j + 2));
+    return celerity(s, dzy(P, g.at(t, i, j)));
+  }
+  // averaged, uncorrected C-grid u (valid for 3 <= i < N-2)
+  __device__ T ucx_pre(int i, int j) const {
+    const int N = g.N;
+    if ((i == 3 || i == N - 3) && j >= 3 && j < N - 3)
+      return evel[(t * 4 + (i == 3 ? DS : DN)) * g.nx + (j - 3)];
+    return T(0.5) * (e5(U(i - 3, j), U(i - 2, j), U(i - 1, j), U(i, j), U(i + 1, j)) +
+                     w5(U(i - 2, j), U(i - 1, j), U(i, j), U(i + 1, j), U(i + 2, j)));
+  }
+  __device__ T vcy_pre(int i, int j) const {
+    const int N = g.N;
+    if ((j == 3 || j == N - 3) && i >= 3 && i < N - 3)
+      return evel[(t * 4 + (j == 3 ? DW : DE)) * g.nx + (i - 3)];
+    return T(0.5) * (e5(V(i, j - 3), V(i, j - 2), V(i, j - 1), V(i, j), V(i, j + 1)) +
+                     w5(V(i, j - 2), V(i, j - 1), V(i, j), V(i, j + 1), V(i, j + 2)));
+  }
+  __device__ T ucx(int i, int j) const {
+    const int N = g.N;
+    if (i < 3 || i >= N - 2) return T(0);
+    const T he = e5(Hh(i - 3, j), Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j));
+    const T hw = w5(Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j), Hh(i + 2, j));
+    return ucx_pre(i, j) + T(4.9) / cx(i, j) * (he - hw);
+  }
+  __device__ T vcy(int i, int j) const {
+    const int N = g.N;
+    if (j < 3 || j >= N - 2) return T(0);
+    const T hn = e5(Hh(i, j - 3), Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1));
+    const T hs = w5(Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1), Hh(i, j + 2));
+    return vcy_pre(i, j) + T(4.9) / cy(i, j) * (hn - hs);
+  }
+  __device__ T vcx(int i, int j) const {
+    const int N = g.N;
+    if (i < 3 || i >= N - 2) return T(0);
+    const size_t b = g.at(t, 0, j);
+    auto Vv = [&](int ii) { return v[b + static_cast<size_t>(ii) * N]; };
+    return ucx_pre(i, j) > T(0) ? e5(Vv(i - 3), Vv(i - 2), Vv(i - 1), Vv(i), Vv(i + 1))
+                                : w5(Vv(i - 2), Vv(i - 1), Vv(i), Vv(i + 1), Vv(i + 2));
+  }
+  __device__ T ucy(int i, int j) const {
+    const int N = g.N;
+    if (j < 3 || j >= N - 2) return T(0);
+    const size_t b = g.at(t, i, 0);
+    auto Uu = [&](int jj) { return u[b + jj]; };
+    return vcy_pre(i, j) > T(0) ? e5(Uu(j - 3), Uu(j - 2), Uu(j - 1), Uu(j), Uu(j + 1))
+                                : w5(Uu(j - 2), Uu(j - 1), Uu(j), Uu(j + 1), Uu(j + 2));
+  }
+  __device__ T zeta(int i, int j) const {
+    const int N = g.N;
+    if (i >= N - 1 || j >= N - 1) return T(0);
+    const size_t o = g.at(t, i, j);
+    return P.rda[o] * (((ucy(i, j) * P.ddx[o] + vcx(i + 1, j) * P.cdy[o + N]) - ucy(i, j + 1) * P.ddx[o + 1]) -
+                       vcx(i, j) * P.cdy[o]);
+  }
+  __device__ T ku(int i, int j) const {
+    const T a = ucx(i, j), b = vcx(i, j);
+    return T(0.5) * (a * a + b * b);
+  }
+  __device__ T kv(int i, int j) const {
+    const T a = vcy(i, j), b = ucy(i, j);
+    return T(0.5) * (a * a + b * b);
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(DiagArgs<T> a) {
+  const Geo g = a.g;
+  const int N = g.N;
+  const size_t NN = static_cast<size_t>(N) * N;
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= 6 * NN) return;
+  const int t = static_cast<int>(q / NN);
+  const int i = static_cast<int>((q % NN) / N);
+  const int j = static_cast<int>(q % N);
+  const DiagCell<T> C{g, a.P, a.h, a.u, a.v, a.evel, t};
+  a.cx[q] = C.cx(i, j);
+  a.cy[q] = C.cy(i, j);
+  const T uct = C.U(i, j), vct = C.V(i, j);
+  a.uct[q] = uct;
+  a.vct[q] = vct;
+  a.ucx[q] = C.ucx(i, j);
+  a.vcy[q] = C.vcy(i, j);
+  a.vcx[q] = C.vcx(i, j);
+  a.ucy[q] = C.ucy(i, j);
+  T ax = T(0), ay = T(0);
+  if (i < N - 1 || j < N - 1) {
+    const T z = C.zeta(i, j);
+    if (i < N - 1) ax = z * vct - (C.ku(i + 1, j) - C.ku(i, j)) * a.P.rdx[q];
+    if (j < N - 1) ay = -z * uct - (C.kv(i, j + 1) - C.kv(i, j)) * a.P.rdy[q];
+  }
+  a.advx[q] = ax;
+  a.advy[q] = ay;
+}
+
+// ---------------------------------------------------------------------------
+// const repacking (binary64 reference layouts -> context planes of type T)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RepackArgs {
+  size_t n;  // cells
+  const double *dzx, *dzy, *rdx, *rdy, *af, *coef, *c2l, *l2c, *gct, *sina, *rda, *cdy, *ddx;
+  const int *li, *lj;
+  T *o_dzx, *o_dzy, *o_rdx, *o_rdy, *o_af, *o_rda, *o_cdy, *o_ddx, *o_sina;
+  T *o_g00, *o_g01, *o_g10, *o_g11, *o_c00, *o_c01, *o_c10, *o_c11, *o_l00, *o_l01, *o_l10, *o_l11;
+  T *o_k0, *o_k1;
+  int *o_li, *o_lj;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_repack(RepackArgs<T> a) {
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.n) return;
+  a.o_dzx[q] = T(a.dzx[q]);
+  a.o_dzy[q] = T(a.dzy[q]);
+  a.o_rdx[q] = T(a.rdx[q]);
+  a.o_rdy[q] = T(a.rdy[q]);
+  a.o_af[q] = T(a.af[q]);
+  a.o_rda[q] = T(a.rda[q]);
+  a.o_cdy[q] = T(a.cdy[q]);
+  a.o_ddx[q] = T(a.ddx[q]);
+  a.o_sina[q] = T(a.sina[q]);
+  a.o_g00[q] = T(a.gct[4 * q + 0]);
+  a.o_g01[q] = T(a.gct[4 * q + 1]);
+  a.o_g10[q] = T(a.gct[4 * q + 2]);
+  a.o_g11[q] = T(a.gct[4 * q + 3]);
+  a.o_c00[q] = T(a.c2l[4 * q + 0]);
+  a.o_c01[q] = T(a.c2l[4 * q + 1]);
+  a.o_c10[q] = T(a.c2l[4 * q + 2]);
+  a.o_c11[q] = T(a.c2l[4 * q + 3]);
+  a.o_l00[q] = T(a.l2c[4 * q + 0]);
+  a.o_l01[q] = T(a.l2c[4 * q + 1]);
+  a.o_l10[q] = T(a.l2c[4 * q + 2]);
+  a.o_l11[q] = T(a.l2c[4 * q + 3]);
+  a.o_k0[q] = T(a.coef[2 * q + 0]);
+  a.o_k1[q] = T(a.coef[2 * q + 1]);
+  a.o_li[q] = a.li[q];
+  a.o_lj[q] = a.lj[q];
+}
+
+constexpr int kNumPlanes = 23;  // T planes in the context (see Planes)
+
+inline unsigned blocks_for(size_t n, unsigned per = 256) {
+  return static_cast<unsigned>((n + per - 1) / per);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct sw_ctx {
+  int nx = 0, N = 0, dtype = SW_F64;
+  hipStream_t stream = nullptr;
+  size_t cells = 0;        // 6 * N * N
+  void* planes = nullptr;  // kNumPlanes T planes
+  int* locs = nullptr;     // loc_i, loc_j planes
+  void* scratch = nullptr; // 9 T fields: S1 (h,u,v), S2 (h,u,v), tmp (h,u,v)
+  void* edges = nullptr;   // evel, eflux, edge-average buffer (24*nx each)
+  double dzx_s = 0, dzy_s = 0;
+  int dz_uniform = 0;
+};
+
+namespace {
+
+constexpr int TA_I = 16, TA_J = 64;  // k_stage_a tile
+constexpr int TB_I = 8, TB_J = 64;   // k_stage_b tile
+
+template <typename T>
+Planes<T> planes_of(const sw_ctx* c) {
+  const T* p = static_cast<const T*>(c->planes);
+  const size_t n = c->cells;
+  Planes<T> P;
+  const T** slots[kNumPlanes] = {&P.dzx, &P.dzy, &P.rdx, &P.rdy, &P.af, &P.rda, &P.cdy, &P.ddx,
+                                 &P.sina, &P.g00, &P.g01, &P.g10, &P.g11, &P.c00, &P.c01, &P.c10,
+                                 &P.c11, &P.l00, &P.l01, &P.l10, &P.l11, &P.k0, &P.k1};
+  for (int k = 0; k < kNumPlanes; ++k) *slots[k] = p + k * n;
+  P.li = c->locs;
+  P.lj = c->locs + n;
+  P.dzx_s = T(c->dzx_s);
+  P.dzy_s = T(c->dzy_s);
+  P.dz_uniform = c->dz_uniform;
+  return P;
+}
+
+template <typename T>
+T* scratch_field(const sw_ctx* c, int k) {
+  return static_cast<T*>(c->scratch) + static_cast<size_t>(k) * c->cells;
+}
+
+template <typename T>
+T* edge_buf(const sw_ctx* c, int k) {
+  return static_cast<T*>(c->edges) + static_cast<size_t>(k) * 24 * c->nx;
+}
+
+Geo geo_of(const sw_ctx* c) { return Geo{c->nx, c->N}; }
+
+template <typename T>
+int launch_halo_scalar(const sw_ctx* c, T* f, int ncomp) {
+  HaloArgs<T> a{geo_of(c), route(), planes_of<T>(c), f, ncomp, nullptr, nullptr};
+  const size_t n = 6 * (12 * static_cast<size_t>(c->nx) + 36);
+  hipLaunchKernelGGL(k_halo_scalar<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+  SW_HIP(hipGetLastError());
+  return SW_OK;
+}
+
+template <typename T>
+int launch_halo_vector(const sw_ctx* c, T* u, T* v) {
+  HaloArgs<T> a{geo_of(c), route(), planes_of<T>(c), nullptr, 1, u, v};
+  const size_t n = 6 * (12 * static_cast<size_t>(c->nx) + 36);
+  hipLaunchKernelGGL(k_halo_vector<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+  SW_HIP(hipGetLastError());
+  return SW_OK;
+}
+
+// One RK stage from (h, hp, u, v, up, vp) into (h_out, u_out, v_out), halos
+// filled.  Leaves the averaged edge velocities of this stage in evel.
+template <typename T>
+int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, const T* up,
+              const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last) {
+  const Geo g = geo_of(c);
+  const Planes<T> P = planes_of<T>(c);
+  T* evel = edge_buf<T>(c, 0);
+  T* eflux = edge_buf<T>(c, 1);
+  EdgeArgs<T> ea{g, route(), P, h, u, v, evel, eflux, last};
+  hipLaunchKernelGGL(k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), dim3(256), 0,
+                     c->stream, ea);
+  SW_HIP(hipGetLastError());
+
+  StageArgs<T> sa{};
+  sa.g = g;
+  sa.P = P;
+  sa.h = h; sa.hp = hp; sa.u = u; sa.v = v; sa.up = up; sa.vp = vp;
+  sa.hn = h_out;
+  sa.evel = evel; sa.eflux = eflux;
+  sa.h_out = h_out; sa.u_out = u_out; sa.v_out = v_out;
+  sa.dt = T(dt);
+  sa.beta = T(beta);
+  sa.last = last;
+  sa.nti = (c->nx + TA_I - 1) / TA_I;
+  sa.ntj = (c->nx + TA_J - 1) / TA_J;
+  hipLaunchKernelGGL((k_stage_a<T, TA_I, TA_J>), dim3(6 * sa.nti * sa.ntj), dim3(kThreads), 0,
+                     c->stream, sa);
+  SW_HIP(hipGetLastError());
+  int rc = launch_halo_scalar<T>(c, h_out, 1);
+  if (rc) return rc;
+  sa.nti = (c->nx + TB_I - 1) / TB_I;
+  sa.ntj = (c->nx + TB_J - 1) / TB_J;
+  hipLaunchKernelGGL((k_stage_b<T, TB_I, TB_J>), dim3(6 * sa.nti * sa.ntj), dim3(kThreads), 0,
+                     c->stream, sa);
+  SW_HIP(hipGetLastError());
+  return launch_halo_vector<T>(c, u_out, v_out);
+}
+
+template <typename T>
+int run_diag(const sw_ctx* c, const T* h, const T* u, const T* v, void* const out[13]) {
+  DiagArgs<T> d{};
+  d.g = geo_of(c);
+  d.P = planes_of<T>(c);
+  d.h = h; d.u = u; d.v = v;
+  d.evel = edge_buf<T>(c, 0);
+  T** dst[10] = {&d.cx, &d.cy, &d.uct, &d.vct, &d.ucx, &d.ucy, &d.vcx, &d.vcy, &d.advx, &d.advy};
+  for (int k = 0; k < 10; ++k) *dst[k] = static_cast<T*>(out[3 + k]);
+  hipLaunchKernelGGL(k_diag<T>, dim3(blocks_for(c->cells)), dim3(256), 0, c->stream, d);
+  SW_HIP(hipGetLastError());
+  return SW_OK;
+}
+
+int check_ptrs(const void* const* p, int n, const char* what) {
+  if (!p) return fail(SW_E_ARG, std::string(what) + " is null");
+  for (int k = 0; k < n; ++k)
+    if (!p[k]) return fail(SW_E_ARG, std::string(what) + "[" + std::to_string(k) + "] is null");
+  return SW_OK;
+}
+
+int check_disjoint(const sw_ctx* c, const void* const in[6], void* const out[13]) {
+  for (int k = 0; k < 13; ++k)
+    for (int m = 0; m < 6; ++m)
+      if (out[k] == in[m]) return fail(SW_E_ARG, "output " + std::to_string(k) + " aliases an input");
+  for (int k = 0; k < 13; ++k)
+    for (int m = k + 1; m < 13; ++m)
+      if (out[k] == out[m]) return fail(SW_E_ARG, "outputs alias each other");
+  (void)c;
+  return SW_OK;
+}
+
+// RK driver: nstage stages per substep with (dt fraction, beta, last).
+template <typename T>
+int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, double dtb, int flags,
+           int rk) {
+  int rc = check_ptrs(in, 6, "in");
+  if (rc) return rc;
+  rc = check_ptrs(const_cast<const void* const*>(out), 13, "out");
+  if (rc) return rc;
+  rc = check_disjoint(c, in, out);
+  if (rc) return rc;
+  if (nbb < 0) return fail(SW_E_ARG, "nbb < 0");
+  const size_t bytes = c->cells * sizeof(T);
+  if (nbb == 0) {
+    SW_HIP(hipMemcpyAsync(out[0], in[0], bytes, hipMemcpyDeviceToDevice, c->stream));
+    SW_HIP(hipMemcpyAsync(out[1], in[2], bytes, hipMemcpyDeviceToDevice, c->stream));
+    SW_HIP(hipMemcpyAsync(out[2], in[3], bytes, hipMemcpyDeviceToDevice, c->stream));
+    return SW_OK;
+  }
+  const T* h = static_cast<const T*>(in[0]);
+  const T* hp = static_cast<const T*>(in[1]);
+  const T* u = static_cast<const T*>(in[2]);
+  const T* v = static_cast<const T*>(in[3]);
+  const T* up = static_cast<const T*>(in[4]);
+  const T* vp = static_cast<const T*>(in[5]);
+  T* s1[3] = {scratch_field<T>(c, 0), scratch_field<T>(c, 1), scratch_field<T>(c, 2)};
+  T* s2[3] = {scratch_field<T>(c, 3), scratch_field<T>(c, 4), scratch_field<T>(c, 5)};
+  T* tmp[3] = {scratch_field<T>(c, 6), scratch_field<T>(c, 7), scratch_field<T>(c, 8)};
+  T* fin[3] = {static_cast<T*>(out[0]), static_cast<T*>(out[1]), static_cast<T*>(out[2])};
+  const T* dh = nullptr;
+  const T* du = nullptr;
+  const T* dv = nullptr;
+  for (int k = 0; k < nbb; ++k) {
+    T** dst = ((nbb - 1 - k) % 2 == 0) ? fin : tmp;
+    if (rk == 2) {
+      rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 2.0, 0.0, 0);
+      if (rc) return rc;
+      rc = run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, dst[0], dst[1], dst[2], dtb, 0.0, 1);
+      if (rc) return rc;
+      dh = s1[0]; du = s1[1]; dv = s1[2];
+    } else {
+      rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 3.0, 1.0, 0);
+      if (rc) return rc;
+      rc = run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, s2[0], s2[1], s2[2], dtb / 2.0, 1.0, 0);
+      if (rc) return rc;
+      rc = run_stage<T>(c, s2[0], hp, s2[1], s2[2], up, vp, dst[0], dst[1], dst[2], dtb, 1.0, 1);
+      if (rc) return rc;
+      dh = s2[0]; du = s2[1]; dv = s2[2];
+    }
+    h = hp = dst[0];
+    u = up = dst[1];
+    v = vp = dst[2];
+  }
+  if (!(flags & SW_SKIP_DIAG)) return run_diag<T>(c, dh, du, dv, out);
+  return SW_OK;
+}
+
+template <typename T>
+int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
+  const size_t n = c->cells;
+  SW_HIP(hipMalloc(&c->planes, kNumPlanes * n * sizeof(T)));
+  SW_HIP(hipMalloc(&c->locs, 2 * n * sizeof(int)));
+  SW_HIP(hipMalloc(&c->scratch, 9 * n * sizeof(T)));
+  SW_HIP(hipMalloc(&c->edges, 3 * 24 * static_cast<size_t>(c->nx) * sizeof(T)));
+  SW_HIP(hipMemsetAsync(c->scratch, 0, 9 * n * sizeof(T), c->stream));
+  SW_HIP(hipMemsetAsync(c->edges, 0, 3 * 24 * static_cast<size_t>(c->nx) * sizeof(T), c->stream));
+  T* p = static_cast<T*>(c->planes);
+  RepackArgs<T> a{};
+  a.n = n;
+  a.dzx = static_cast<const double*>(consts[0]);
+  a.dzy = static_cast<const double*>(consts[1]);
+  a.rdx = static_cast<const double*>(consts[2]);
+  a.rdy = static_cast<const double*>(consts[3]);
+  a.af = static_cast<const double*>(consts[4]);
+  a.coef = static_cast<const double*>(consts[5]);
+  a.li = static_cast<const int*>(consts[6]);
+  a.lj = static_cast<const int*>(consts[7]);
+  a.c2l = static_cast<const double*>(consts[8]);
+  a.l2c = static_cast<const double*>(consts[9]);
+  a.gct = static_cast<const double*>(consts[12]);
+  a.sina = static_cast<const double*>(consts[13]);
+  a.rda = static_cast<const double*>(consts[14]);
+  a.cdy = static_cast<const double*>(consts[15]);
+  a.ddx = static_cast<const double*>(consts[16]);
+  T** outs[kNumPlanes] = {&a.o_dzx, &a.o_dzy, &a.o_rdx, &a.o_rdy, &a.o_af, &a.o_rda, &a.o_cdy, &a.o_ddx,
+                          &a.o_sina, &a.o_g00, &a.o_g01, &a.o_g10, &a.o_g11, &a.o_c00, &a.o_c01, &a.o_c10,
+                          &a.o_c11, &a.o_l00, &a.o_l01, &a.o_l10, &a.o_l11, &a.o_k0, &a.o_k1};
+  for (int k = 0; k < kNumPlanes; ++k) *outs[k] = p + k * n;
+  a.o_li = c->locs;
+  a.o_lj = c->locs + n;
+  hipLaunchKernelGGL(k_repack<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+  SW_HIP(hipGetLastError());
+  return SW_OK;
+}
+
+// Host-side validation of the tables the kernels index with.
+int validate_consts(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
+  const size_t n = c->cells;
+  const int N = c->N;
+  std::vector<int> li(n), lj(n);
+  std::vector<double> inner(n), outer(n), dzx(n), dzy(n);
+  SW_HIP(hipMemcpy(li.data(), consts[6], n * sizeof(int), hipMemcpyDeviceToHost));
+  SW_HIP(hipMemcpy(lj.data(), consts[7], n * sizeof(int), hipMemcpyDeviceToHost));
+  SW_HIP(hipMemcpy(inner.data(), consts[10], n * sizeof(double), hipMemcpyDeviceToHost));
+  SW_HIP(hipMemcpy(outer.data(), consts[11], n * sizeof(double), hipMemcpyDeviceToHost));
+  SW_HIP(hipMemcpy(dzx.data(), consts[0], n * sizeof(double), hipMemcpyDeviceToHost));
+  SW_HIP(hipMemcpy(dzy.data(), consts[1], n * sizeof(double), hipMemcpyDeviceToHost));
+  for (int t = 0; t < 6; ++t) {
+    for (int i = 0; i < N; ++i) {
+      for (int j = 0; j < N; ++j) {
+        const size_t o = (static_cast<size_t>(t) * N + i) * N + j;
+        const bool in = i >= 3 && i < N - 3 && j >= 3 && j < N - 3;
+        if (inner[o] != (in ? 1.0 : 0.0) || outer[o] != (in ? 0.0 : 1.0))
+          return fail(SW_E_CONST, "inner/outer are not the standard interior/halo masks");
+        const bool ew = (j < 3 || j >= N - 3) && i >= 3 && i < N - 3;
+        const bool ns = (i < 3 || i >= N - 3) && j >= 3 && j < N - 3;
+        if (ew && (li[o] < 3 || li[o] > N - 5))
+          return fail(SW_E_CONST, "loc_i_local out of range on a W/E halo strip");
+        if (ns && (lj[o] < 3 || lj[o] > N - 5))
+          return fail(SW_E_CONST, "loc_j_local out of range on a S/N halo strip");
+      }
+    }
+  }
+  bool uni = true;
+  for (size_t o = 0; o < n && uni; ++o) uni = (dzx[o] == dzx[0]) && (dzy[o] == dzy[0]);
+  c->dz_uniform = uni ? 1 : 0;
+  c->dzx_s = dzx[0];
+  c->dzy_s = dzy[0];
+  return SW_OK;
+}
+
+void free_ctx(sw_ctx* c) {
+  if (!c) return;
+  if (c->planes) (void)hipFree(c->planes);
+  if (c->locs) (void)hipFree(c->locs);
+  if (c->scratch) (void)hipFree(c->scratch);
+  if (c->edges) (void)hipFree(c->edges);
+  delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CONSTS], void* stream) {
+  if (!out) return fail(SW_E_ARG, "out is null");
+  *out = nullptr;
+  if (nx < 7 || nx > 8192) return fail(SW_E_ARG, "nx must be in [7, 8192]");
+  if (dtype != SW_F64 && dtype != SW_F32) return fail(SW_E_ARG, "dtype must be SW_F64 or SW_F32");
+  int rc = check_ptrs(consts, SW_N_CONSTS, "consts");
+  if (rc) return rc;
+  sw_ctx* c = new sw_ctx();
+  c->nx = nx;
+  c->N = nx + 2 * HALO;
+  c->dtype = dtype;
+  c->stream = static_cast<hipStream_t>(stream);
+  c->cells = 6 * static_cast<size_t>(c->N) * c->N;
+  rc = validate_consts(c, consts);
+  if (!rc) rc = (dtype == SW_F64) ? do_create_planes<double>(c, consts) : do_create_planes<float>(c, consts);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(SW_E_HIP, std::string("sw_create: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return SW_OK;
+}
+
+void sw_destroy(sw_ctx* ctx) {
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  free_ctx(ctx);
+}
+
+int sw_set_stream(sw_ctx* ctx, void* stream) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return SW_OK;
+}
+
+int sw_rk2(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb, double dtb, int flags) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  return ctx->dtype == SW_F64 ? run_rk<double>(ctx, in, out, nbb, dtb, flags, 2)
+                              : run_rk<float>(ctx, in, out, nbb, dtb, flags, 2);
+}
+
+int sw_rk3(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb, double dtb, int flags) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  return ctx->dtype == SW_F64 ? run_rk<double>(ctx, in, out, nbb, dtb, flags, 3)
+                              : run_rk<float>(ctx, in, out, nbb, dtb, flags, 3);
+}
+
+int sw_stage(sw_ctx* ctx, const void* const in[6], void* const out[13], double dt, double beta, int is_last) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  int rc = check_ptrs(in, 6, "in");
+  if (rc) return rc;
+  rc = check_ptrs(const_cast<const void* const*>(out), 13, "out");
+  if (rc) return rc;
+  rc = check_disjoint(ctx, in, out);
+  if (rc) return rc;
+  if (ctx->dtype == SW_F64) {
+    auto p = [&](int k) { return static_cast<const double*>(in[k]); };
+    rc = run_stage<double>(ctx, p(0), p(1), p(2), p(3), p(4), p(5), static_cast<double*>(out[0]),
+                           static_cast<double*>(out[1]), static_cast<double*>(out[2]), dt, beta, is_last ? 1 : 0);
+    if (rc) return rc;
+    return run_diag<double>(ctx, p(0), p(2), p(3), out);
+  }
+  auto p = [&](int k) { return static_cast<const float*>(in[k]); };
+  rc = run_stage<float>(ctx, p(0), p(1), p(2), p(3), p(4), p(5), static_cast<float*>(out[0]),
+                        static_cast<float*>(out[1]), static_cast<float*>(out[2]), dt, beta, is_last ? 1 : 0);
+  if (rc) return rc;
+  return run_diag<float>(ctx, p(0), p(2), p(3), out);
+}
+
+int sw_halo_scalar(sw_ctx* ctx, void* field, int ncomp) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  if (!field) return fail(SW_E_ARG, "field is null");
+  if (ncomp < 1 || ncomp > 64) return fail(SW_E_ARG, "ncomp must be in [1, 64]");
+  return ctx->dtype == SW_F64 ? launch_halo_scalar<double>(ctx, static_cast<double*>(field), ncomp)
+                              : launch_halo_scalar<float>(ctx, static_cast<float*>(field), ncomp);
+}
+
+int sw_halo_vector(sw_ctx* ctx, void* u, void* v) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  if (!u || !v || u == v) return fail(SW_E_ARG, "u, v must be distinct non-null fields");
+  return ctx->dtype == SW_F64 ? launch_halo_vector<double>(ctx, static_cast<double*>(u), static_cast<double*>(v))
+                              : launch_halo_vector<float>(ctx, static_cast<float*>(u), static_cast<float*>(v));
+}
+
+int sw_edge_avg(sw_ctx* ctx, void* u, void* v) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  if (!u || !v || u == v) return fail(SW_E_ARG, "u, v must be distinct non-null fields");
+  const size_t n = 24 * static_cast<size_t>(ctx->nx);
+  if (ctx->dtype == SW_F64) {
+    EdgeAvgArgs<double> a{geo_of(ctx), route(), static_cast<double*>(u), static_cast<double*>(v),
+                          edge_buf<double>(ctx, 2)};
+    hipLaunchKernelGGL(k_edge_avg_read<double>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_edge_avg_write<double>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
+  } else {
+    EdgeAvgArgs<float> a{geo_of(ctx), route(), static_cast<float*>(u), static_cast<float*>(v),
+                         edge_buf<float>(ctx, 2)};
+    hipLaunchKernelGGL(k_edge_avg_read<float>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_edge_avg_write<float>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
+  }
+  SW_HIP(hipGetLastError());
+  return SW_OK;
+}
+
+int sw_sync(sw_ctx* ctx) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  SW_HIP(hipStreamSynchronize(ctx->stream));
+  return SW_OK;
+}
+
+int sw_routing_table(int32_t out[6 * 4 * 3]) {
+  if (!out) return fail(SW_E_ARG, "out is null");
+  const Route& R = route();
+  for (int t = 0; t < 6; ++t)
+    for (int d = 0; d < 4; ++d)
+      for (int k = 0; k < 3; ++k) out[(t * 4 + d) * 3 + k] = R.r[t][d][k];
+  return SW_OK;
+}
+
+int sw_info(const sw_ctx* ctx, int* nx, int* dtype) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  if (nx) *nx = ctx->nx;
+  if (dtype) *dtype = ctx->dtype;
+  return SW_OK;
+}
+
+const char* sw_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"
