@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the barotropic RK2 step on one MI355X.
+
+A *step* is one drop-in call `barotr_rk2_core(state, consts, nbb, dtb)` on the
+C768 FP64 duo-grid (6 x 768^2 = 3,538,944 interior cells) with nbb = 2000
+substeps, the substep pairs replaying recorded hipGraphs.  Inputs are resident
+in HBM before the timed region.  The reported value is interior cells x
+substeps / wall time (cell-substeps/s).
+
+The JSON line also carries
+  roofline       the dominant kernel's algorithmic HBM bytes per launch over
+                 its average launch time, measured here with hipEvents on the
+                 launch stream (eager pass of the same workload),
+  roofline_path  the whole step against 296 B per FP64 cell-substep (148 B for
+                 FP32) at 8 TB/s,
+  cpu_baseline   the NumPy FP64 oracle on one host core over a bounded sample
+                 of the same workload.
+Only one GPU is used: --gpus above 1 exits with a message.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "cell-substeps/s + % HBM roofline at C768; p50/p99 latency of served solves"
+
+# Algorithmic bytes per interior cell per launch of the stage kernels: each
+# array the kernel must touch, read or written once, averaged over the two RK2
+# stages (stage 1 reads h0p/ubp/vbp at the same addresses as h0/ub/vb).  The
+# five face-independent metric planes (rdx, rdy, rda, c_dy, d_dx) count 1/6 per
+# cell; a_gct counts two planes (symmetric, equal diagonal).
+#   height kernel : h u v [hp] g00 g01 sina + rda c_dy d_dx/6 + write h  = 7.5 / 8.5
+#   velocity kernel: h hn u v [up vp] g00 g01 sina a_f + 5/6 + write u v = 10.83 / 12.83
+VALUES_PER_CELL = {"march_a": 8.0, "march_b": 11.8333}
+PATH_BYTES = {"f64": 296.0, "f32": 148.0}
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--nx", type=int, default=768)
+    p.add_argument("--nbb", type=int, default=2000, help="substeps per step")
+    p.add_argument("--dtype", choices=("f64", "f32"), default="f64")
+    p.add_argument("--no-graph", action="store_true", help="launch every substep eagerly")
+    p.add_argument("--profile-substeps", type=int, default=20,
+                   help="substeps of the eager event-timed pass (0 disables)")
+    p.add_argument("--cpu-substeps", type=int, default=2,
+                   help="substeps of the CPU oracle sample (0 disables)")
+    p.add_argument("--cpu-nx", type=int, default=None, help="resolution of the CPU sample (default --nx)")
+    return p.parse_args()
+
+
+def cpu_baseline(nx: int, substeps: int):
+    """Time the NumPy FP64 oracle on one core for `substeps` RK2 substeps."""
+    import numpy as np  # noqa: F401
+    from licom4py_gpu_amd import duogrid, initial
+    from oracle import barotr_np
+
+    g = duogrid.grid(nx)
+    h, u, v, sp = initial.seeded_balanced(g)
+    dt = initial.courant_dt(g, h, sp)
+    consts = duogrid.barotr_consts(g, 5000.0)
+    st = initial.state_tuple(h, u, v)
+    t0 = time.perf_counter()
+    barotr_np.barotr_rk2_core(st, consts, substeps, dt)
+    el = time.perf_counter() - t0
+    cells = 6 * nx * nx
+    return {
+        "value": cells * substeps / el,
+        "unit": "cell-substeps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"C{nx} FP64 RK2, {substeps} substeps of the same seeded field, NumPy oracle "
+                  f"(single-threaded) on the GPU host: {el:.2f} s",
+    }
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main() -> int:
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) "
+              "is not supported", file=sys.stderr)
+        return 2
+
+    import torch
+    from licom4py_gpu_amd import barotr, duogrid, initial
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    nx, nbb = args.nx, args.nbb
+    tdt = torch.float64 if args.dtype == "f64" else torch.float32
+
+    t_setup = time.perf_counter()
+    g = duogrid.grid(nx)
+    h, u, v, speed = initial.seeded_balanced(g)
+    dt = initial.courant_dt(g, h, speed, depth=5000.0)
+    consts = barotr.as_device_consts(duogrid.barotr_consts(g, 5000.0), dev)
+    state = tuple(torch.as_tensor(x).to(device=dev, dtype=tdt) for x in initial.state_tuple(h, u, v))
+    ctx = barotr.context_for(consts, tdt)
+    setup_s = time.perf_counter() - t_setup
+
+    def step(s):
+        return barotr.barotr_rk2_core(s, consts, nbb, dt) if not args.no_graph else \
+            _eager(ctx, s, nbb, dt)
+
+    s = state
+    for _ in range(args.warmup):
+        s = step(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s = step(s)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    hfin = s[0][:, 3:-3, 3:-3]
+    if not bool(torch.isfinite(hfin).all()):
+        print("bench.py: non-finite height after the timed steps", file=sys.stderr)
+        return 1
+
+    cells = 6 * nx * nx
+    value = cells * nbb * args.steps / el
+    ms_per_step = el / args.steps * 1e3
+    path_bytes = PATH_BYTES[args.dtype]
+    workload = f"C{nx}_{args.dtype}"
+
+    roofline = None
+    kernels = None
+    if args.profile_substeps > 0:
+        prof = ctx.profile(state[:6], args.profile_substeps, dt)
+        kernels = {k: {"ms_total": round(ms, 4), "launches": n,
+                       "us_avg": round(ms / n * 1e3, 3) if n else None}
+                   for k, (ms, n) in prof.items()}
+        dom = max(("march_a", "march_b"), key=lambda k: prof[k][0])
+        ms, n = prof[dom]
+        avg_s = ms / n * 1e-3
+        esize = 8 if args.dtype == "f64" else 4
+        alg = VALUES_PER_CELL[dom] * esize * cells
+        achieved = alg / avg_s / 1e9
+        traffic = load_traffic(dom, workload)
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                    "traffic": traffic, "algorithmic_bytes_per_launch": int(alg),
+                    "avg_launch_us": round(avg_s * 1e6, 3)}
+    path_ach = value * path_bytes / 1e9
+    roofline_path = {"bound": "hbm", "achieved": round(path_ach, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(path_ach / PEAK_HBM_GBS, 4),
+                     "bytes_per_cell_substep": path_bytes}
+
+    cpu = None
+    if args.cpu_substeps > 0:
+        cpu = cpu_baseline(args.cpu_nx or nx, args.cpu_substeps)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic: seeded balanced spherical-harmonic height (degree 1-8, max 200 m) with "
+                "geostrophic velocity on a generated equal-edge duo-grid, depth 5000 m, Courant 0.5",
+        "config": {"workload": f"C{nx} {args.dtype.upper()} RK2 barotropic step, {nbb} substeps per step"
+                               + ("" if args.no_graph else ", hipGraph substep pairs"),
+                   "nx": nx, "interior_cells": cells, "substeps_per_step": nbb,
+                   "dt_s": dt, "setup_s": round(setup_s, 2)},
+        "roofline": roofline,
+        "roofline_path": roofline_path,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _eager(ctx, s, nbb, dt):
+    out = ctx.rk(s[:6], nbb, dt, graph=False)
+    h, u, v = out[0], out[1], out[2]
+    return (h, h, u, v, u, v, *out[3:13]) + tuple(s[16:23])
+
+
+if __name__ == "__main__":
+    sys.exit(main())

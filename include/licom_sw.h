@@ -48,7 +48,9 @@ enum {
 
 /* Flags for sw_rk2 / sw_rk3. */
 enum {
-  SW_SKIP_DIAG = 1   /* do not write the 10 diagnostic outputs */
+  SW_SKIP_DIAG = 1,  /* do not write the 10 diagnostic outputs */
+  SW_GRAPH = 2       /* replay recorded hipGraphs of substep pairs (same
+                        results; the first call per dtb records them) */
 };
 
 /* Number of consts and their order: mymodule/schedule.py:95-116
@@ -103,6 +105,24 @@ int sw_sync(sw_ctx* ctx);
 /* Host-only: the cube routing table R[t][d] = (neighbour face, its edge,
  * transform 0=ID / 1=FLIP), directions E=0, W=1, N=2, S=3. */
 int sw_routing_table(int32_t out[6 * 4 * 3]);
+
+/* Eager RK2 run of nbb substeps (as sw_rk2 with flags 0) with a hipEvent
+ * pair around every launch, on the context's stream; waits for completion.
+ * ms[k] / counts[k] = total milliseconds / launches of kernel class k:
+ * 0 edge lines, 1 height stage kernel, 2 height halo fill, 3 velocity stage
+ * kernel, 4 velocity halo fill, 5 diagnostics.  A measurement helper with no
+ * counterpart in the reference. */
+int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
+               double dtb, double ms[6], int counts[6]);
+
+/* Rows per wavefront band of the two stage kernels (performance only; any
+ * value in [4, 4096] gives the same results). */
+int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b);
+
+/* Host-only: what sw_create detected in the consts.  Bit 0: dzph_x/dzph_y
+ * uniform (read as scalars); bit 1: rdx, rdy, rda, c_dy, d_dx identical on all
+ * faces (read from face 0); bit 2: a_gct symmetric with equal diagonal. */
+int sw_layout_flags(const sw_ctx* ctx, int* flags);
 
 /* Host-only: resolution and dtype of a context. */
 int sw_info(const sw_ctx* ctx, int* nx, int* dtype);

@@ -15,13 +15,15 @@ from . import build as _build
 SW_F64 = 0
 SW_F32 = 1
 SW_SKIP_DIAG = 1
+SW_GRAPH = 2
+KERNEL_CLASSES = ("edge", "march_a", "halo_h", "march_b", "halo_uv", "diag")
 SW_N_CONSTS = 17
 
 # Every symbol declared in include/licom_sw.h.
 EXPORTED = (
     "sw_create", "sw_destroy", "sw_set_stream", "sw_rk2", "sw_rk3", "sw_stage",
     "sw_halo_scalar", "sw_halo_vector", "sw_edge_avg", "sw_sync",
-    "sw_routing_table", "sw_info", "sw_last_error",
+    "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_layout_flags", "sw_profile",
 )
 
 _lib = None
@@ -65,6 +67,13 @@ def lib() -> ctypes.CDLL:
     L.sw_routing_table.restype = ci
     L.sw_info.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.sw_info.restype = ci
+    L.sw_profile.argtypes = [vp, vpp, vpp, ci, ctypes.c_double,
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ci)]
+    L.sw_profile.restype = ci
+    L.sw_set_tuning.argtypes = [vp, ci, ci]
+    L.sw_set_tuning.restype = ci
+    L.sw_layout_flags.argtypes = [vp, ctypes.POINTER(ci)]
+    L.sw_layout_flags.restype = ci
     L.sw_last_error.argtypes = []
     L.sw_last_error.restype = ctypes.c_char_p
     _lib = L

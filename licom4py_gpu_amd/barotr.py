@@ -122,16 +122,40 @@ class SWContext:
 
     # -- entry points --------------------------------------------------------
     def rk(self, core6: Sequence[torch.Tensor], nbb: int, dtb: float, rk: int = 2,
-           diagnostics: bool = True, out=None):
+           diagnostics: bool = True, out=None, graph: bool = True):
+        """nbb substeps; returns the 13 output tensors (h, u, v, 10 diagnostics).
+        With ``graph`` the substep pairs replay recorded hipGraphs."""
         ins = [self._check_field(x, STATE_KEYS[k]) for k, x in enumerate(core6)]
         outs = out if out is not None else self.empty(13)
         self._bind_stream()
         pin, _k1 = _hip.ptr_array([x.data_ptr() for x in ins])
         pout, _k2 = _hip.ptr_array([x.data_ptr() for x in outs])
         fn = _hip.lib().sw_rk2 if rk == 2 else _hip.lib().sw_rk3
-        flags = 0 if diagnostics else _hip.SW_SKIP_DIAG
+        flags = (0 if diagnostics else _hip.SW_SKIP_DIAG) | (_hip.SW_GRAPH if graph else 0)
         _hip.check(fn(self._h, pin, pout, int(nbb), float(dtb), flags), f"sw_rk{rk}")
         return outs
+
+    def profile(self, core6: Sequence[torch.Tensor], nbb: int, dtb: float):
+        """Eager RK2 run with a hipEvent pair around every launch (on the
+        launch stream); returns {kernel class: (total ms, launches)}."""
+        ins = [self._check_field(x, STATE_KEYS[k]) for k, x in enumerate(core6)]
+        outs = self.empty(13)
+        self._bind_stream()
+        pin, _k1 = _hip.ptr_array([x.data_ptr() for x in ins])
+        pout, _k2 = _hip.ptr_array([x.data_ptr() for x in outs])
+        ms = (ctypes.c_double * 6)()
+        cnt = (ctypes.c_int * 6)()
+        _hip.check(_hip.lib().sw_profile(self._h, pin, pout, int(nbb), float(dtb), ms, cnt), "sw_profile")
+        return {name: (ms[k], cnt[k]) for k, name in enumerate(_hip.KERNEL_CLASSES)}
+
+    def set_tuning(self, rows_a: int, rows_b: int) -> None:
+        _hip.check(_hip.lib().sw_set_tuning(self._h, int(rows_a), int(rows_b)), "sw_set_tuning")
+
+    def layout_flags(self) -> dict:
+        f = ctypes.c_int()
+        _hip.check(_hip.lib().sw_layout_flags(self._h, ctypes.byref(f)), "sw_layout_flags")
+        return {"dz_uniform": bool(f.value & 1), "shared_metrics": bool(f.value & 2),
+                "gct_symmetric": bool(f.value & 4)}
 
     def stage(self, h, hp, u, v, up, vp, dt: float, beta: float, last: bool):
         ins = [self._check_field(x, n) for x, n in zip((h, hp, u, v, up, vp), STATE_KEYS[:6])]

@@ -275,262 +275,436 @@ __global__ __launch_bounds__(256) void k_edge(EdgeArgs<T> a) {
 // ---------------------------------------------------------------------------
 // stage kernels
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Column-marching stage kernels.
+//
+// One wavefront owns a strip of 64 consecutive columns j = c0 + lane of one
+// face and marches down a band of R interior rows.  Values along i (the
+// stencil direction of the i-faces) live in per-lane register windows that
+// slide one row per step; values along j come from a 69-entry LDS row buffer
+// shared by the wave (its own 64 columns plus 3 columns on the left and 2 on
+// the right, loaded by lanes 0..4).  A wave computes j-faces at its 64
+// columns and writes cells c0 .. c0+62, so strips start every 63 columns.
+// The next row's loads are issued one step ahead of their use.  Waves are
+// independent (no workgroup barrier); the six faces of one (strip, band) are
+// adjacent wave ids, so the face-independent metric planes they share are
+// served from cache.
+// ---------------------------------------------------------------------------
+constexpr int kStripOut = 63;  // cells written per wave and row
+constexpr int kRowBuf = 72;    // LDS row buffer length (69 used)
+
 template <typename T>
-struct StageArgs {
+struct MarchArgs {
   Geo g;
   Planes<T> P;
   const T *h, *hp, *u, *v, *up, *vp;  // stage inputs
-  const T *hn;                          // new height with filled halo (stage B)
+  const T *hn;                          // new height, halo filled (kernel B)
   const T *evel, *eflux;
   T *h_out, *u_out, *v_out;
   T dt, beta;
   int last;
-  int nti, ntj;
+  int S, R, nbands, nwaves;
+  int shared;  // rdx, rdy, rda, c_dy, d_dx identical on every face
+  int gsym;    // a_gct symmetric with equal diagonal
 };
 
-constexpr int kThreads = 256;
-
-template <int TI, int TJ>
-__device__ __forceinline__ void tile_origin(int nti, int ntj, int& t, int& i0, int& j0) {
-  const int per_face = nti * ntj;
-  const int b = blockIdx.x;
-  t = b / per_face;
-  const int rem = b - t * per_face;
-  const int ti = rem / ntj;
-  const int tj = rem - ti * ntj;
-  i0 = HALO + ti * TI;
-  j0 = HALO + tj * TJ;
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// New height on the interior of one TI x TJ tile.
-template <typename T, int TI, int TJ>
-__global__ __launch_bounds__(kThreads) void k_stage_a(StageArgs<T> a) {
-  constexpr int HR = TI + 6, HC = TJ + 6, FJ = TJ + 1;
-  __shared__ T s_h[HR * HC];
-  __shared__ T s_uct[HR * TJ];
-  __shared__ T s_vct[TI * HC];
-  __shared__ T s_F[(TI + 1) * TJ];
-  __shared__ T s_G[TI * FJ];
+struct WaveTile {
+  int t, lane, wslot, c0, j, jl, r0, r1;
+  size_t fo, mo;
+};
 
-  const Geo g = a.g;
-  const int N = g.N, nx = g.nx;
-  int t, i0, j0;
-  tile_origin<TI, TJ>(a.nti, a.ntj, t, i0, j0);
-  const int ni = min(TI, N - HALO - i0), nj = min(TJ, N - HALO - j0);
+template <typename T>
+__device__ __forceinline__ bool decode_wave(const MarchArgs<T>& a, WaveTile& w) {
+  const int N = a.g.N;
+  w.lane = threadIdx.x & 63;
+  w.wslot = threadIdx.x >> 6;
+  const int wid = blockIdx.x * (blockDim.x >> 6) + w.wslot;
+  if (wid >= a.nwaves) return false;
+  w.t = wid % 6;
+  const int rest = wid / 6;
+  const int band = rest % a.nbands;
+  const int s = rest / a.nbands;
+  w.c0 = HALO + kStripOut * s;
+  w.j = w.c0 + w.lane;
+  w.jl = min(w.j, N - 1);
+  w.r0 = HALO + band * a.R;
+  w.r1 = min(w.r0 + a.R, N - HALO);
+  const size_t NN = static_cast<size_t>(N) * N;
+  w.fo = static_cast<size_t>(w.t) * NN;
+  w.mo = a.shared ? 0 : w.fo;
+  return true;
+}
+
+// Raw per-point inputs of the contravariant velocity (remap.py:35-39).
+template <typename T>
+struct CtRaw {
+  T u, v, g00, g01, g10, g11, sina;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_ct(const MarchArgs<T>& a, size_t o, CtRaw<T>& r) {
   const Planes<T>& P = a.P;
+  r.u = a.u[o];
+  r.v = a.v[o];
+  r.g00 = P.g00[o];
+  r.g01 = P.g01[o];
+  r.sina = P.sina[o];
+  if (a.gsym) {
+    r.g10 = r.g01;
+    r.g11 = r.g00;
+  } else {
+    r.g10 = P.g10[o];
+    r.g11 = P.g11[o];
+  }
+}
+template <typename T>
+__device__ __forceinline__ T ct_u(const CtRaw<T>& r) { return (r.g00 * r.u + r.g01 * r.v) * r.sina; }
+template <typename T>
+__device__ __forceinline__ T ct_v(const CtRaw<T>& r) { return (r.g10 * r.u + r.g11 * r.v) * r.sina; }
 
-  for (int q = threadIdx.x; q < HR * HC; q += kThreads) {
-    const int r = q / HC, c = q - r * HC;
-    if (r < ni + 6 && c < nj + 6) s_h[q] = a.h[g.at(t, i0 - 3 + r, j0 - 3 + c)];
-  }
-  for (int q = threadIdx.x; q < HR * TJ; q += kThreads) {
-    const int r = q / TJ, c = q - r * TJ;
-    if (r < ni + 6 && c < nj) s_uct[q] = uct_at(P, a.u, a.v, g.at(t, i0 - 3 + r, j0 + c));
-  }
-  for (int q = threadIdx.x; q < TI * HC; q += kThreads) {
-    const int r = q / HC, c = q - r * HC;
-    if (r < ni && c < nj + 6) s_vct[q] = vct_at(P, a.u, a.v, g.at(t, i0 + r, j0 - 3 + c));
-  }
-  __syncthreads();
+// Mass flux through one face from its 6-point column of h and contravariant
+// velocity (barotr_pure.py:29-49; remap.py:42-55): s[] and w[] hold samples
+// k-3 .. k+2 across the face, upre_edge is used on a cube edge.
+template <typename T>
+__device__ __forceinline__ T face_flux(const T* s, const T* w, bool edge, T upre_edge, T dz) {
+  const T he = e5(s[0], s[1], s[2], s[3], s[4]);
+  const T hw = w5(s[1], s[2], s[3], s[4], s[5]);
+  const T upre = edge ? upre_edge
+                      : T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
+  const T c = celerity(c6(s[0], s[1], s[2], s[3], s[4], s[5]), dz);
+  const T uc = upre + T(4.9) / c * (he - hw);
+  return dz * uc + (uc > T(0) ? he : hw) * uc;
+}
 
-  // mass flux through the i-faces (rows i0 .. i0+ni)
-  for (int q = threadIdx.x; q < (TI + 1) * TJ; q += kThreads) {
-    const int k = q / TJ, c = q - k * TJ;
-    if (k > ni || c >= nj) continue;
-    const int i = i0 + k, j = j0 + c;
-    const bool edge = (i == HALO) || (i == N - HALO);
-    const int eidx = (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO);
-    T F;
-    if (a.last && edge) {
-      F = a.eflux[eidx];
-    } else {
-      const T* hc = &s_h[k * HC + c + 3];
-      const T he = e5(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC]);
-      const T hw = w5(hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]);
-      T upre;
-      if (edge) {
-        upre = a.evel[eidx];
-      } else {
-        const T* uc = &s_uct[k * TJ + c];
-        upre = T(0.5) * (e5(uc[0], uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ]) +
-                         w5(uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ], uc[5 * TJ]));
+// Kernel A: new height on the interior (barotr_pure.py:111-120).
+template <typename T>
+__global__ __launch_bounds__(256) void k_march_a(MarchArgs<T> a) {
+  __shared__ T s_v[4][kRowBuf];
+  __shared__ T s_h[4][kRowBuf];
+  WaveTile w;
+  if (!decode_wave(a, w)) return;
+  const int N = a.g.N, nx = a.g.nx;
+  const Planes<T>& P = a.P;
+  T* bv = s_v[w.wslot];
+  T* bh = s_h[w.wslot];
+  const int lane = w.lane, j = w.j, jl = w.jl, c0 = w.c0, t = w.t;
+  const size_t NS = static_cast<size_t>(N);
+
+  // lanes 0..4 also supply the row-buffer columns outside the strip
+  int hcol = lane < 3 ? c0 - 3 + lane : c0 + 61 + lane;
+  hcol = min(hcol, N - 1);
+  const bool halo_lane = lane < 5;
+  const int hslot = hcol - c0 + 3;
+
+  T wu[6], wvc[6], wh[6];
+  for (int m = 0; m < 5; ++m) {
+    const size_t o = w.fo + static_cast<size_t>(w.r0 - 3 + m) * NS + jl;
+    CtRaw<T> r;
+    load_ct(a, o, r);
+    wu[m] = ct_u(r);
+    wvc[m] = ct_v(r);
+    wh[m] = a.h[o];
+  }
+  // prefetch registers: column point of row i+2, metrics of row i / cell row i-1
+  CtRaw<T> nr;
+  T nh;
+  {
+    const size_t o = w.fo + static_cast<size_t>(w.r0 + 2) * NS + jl;
+    load_ct(a, o, nr);
+    nh = a.h[o];
+  }
+  T n_cdy = P.cdy[w.mo + static_cast<size_t>(w.r0) * NS + jl];
+  T n_rda = T(0), n_ddx = T(0), n_hp = T(0), n_dzy = a.P.dzy_s;
+  CtRaw<T> nhr;
+  T nhh = T(0);
+  T Fprev = T(0), cdy_prev = T(0);
+
+  for (int k = 0; k <= w.r1 - w.r0; ++k) {
+    const int i = w.r0 + k;
+    // consume the prefetched row i+2
+    wu[5] = ct_u(nr);
+    wvc[5] = ct_v(nr);
+    wh[5] = nh;
+    const T cdy_i = n_cdy;
+    const T rda_c = n_rda, ddx_c = n_ddx, hp_c = n_hp, dzy_c = n_dzy;
+    const CtRaw<T> hr = nhr;
+    const T hh = nhh;
+    // issue the loads of the next step
+    {
+      const int row = min(i + 3, N - 1);
+      const size_t o = w.fo + static_cast<size_t>(row) * NS + jl;
+      load_ct(a, o, nr);
+      nh = a.h[o];
+      const int mrow = min(i + 1, N - 1);
+      n_cdy = P.cdy[w.mo + static_cast<size_t>(mrow) * NS + jl];
+      const size_t oc = static_cast<size_t>(i) * NS + jl;
+      n_rda = P.rda[w.mo + oc];
+      n_ddx = P.ddx[w.mo + oc];
+      n_hp = a.hp[w.fo + oc];
+      if (!P.dz_uniform) n_dzy = P.dzy[w.fo + oc];
+      if (halo_lane) {
+        const size_t oh = w.fo + static_cast<size_t>(i) * NS + hcol;
+        load_ct(a, oh, nhr);
+        nhh = a.h[oh];
       }
-      const size_t o = g.at(t, i, j);
-      const T dz = dzx(P, o);
-      const T cx = celerity(c6(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]), dz);
-      const T uc = upre + T(4.9) / cx * (he - hw);
-      F = dz * uc + (uc > T(0) ? he : hw) * uc;
     }
-    s_F[q] = F;
-  }
-  // mass flux through the j-faces (columns j0 .. j0+nj)
-  for (int q = threadIdx.x; q < TI * FJ; q += kThreads) {
-    const int k = q / FJ, c = q - k * FJ;
-    if (k >= ni || c > nj) continue;
-    const int i = i0 + k, j = j0 + c;
-    const bool edge = (j == HALO) || (j == N - HALO);
-    const int eidx = (t * 4 + (j == HALO ? DW : DE)) * nx + (i - HALO);
-    T G;
-    if (a.last && edge) {
-      G = a.eflux[eidx];
-    } else {
-      const T* hr = &s_h[(k + 3) * HC + c];
-      const T hn = e5(hr[0], hr[1], hr[2], hr[3], hr[4]);
-      const T hs = w5(hr[1], hr[2], hr[3], hr[4], hr[5]);
-      T vpre;
-      if (edge) {
-        vpre = a.evel[eidx];
-      } else {
-        const T* vc = &s_vct[k * HC + c];
-        vpre = T(0.5) * (e5(vc[0], vc[1], vc[2], vc[3], vc[4]) + w5(vc[1], vc[2], vc[3], vc[4], vc[5]));
-      }
-      const size_t o = g.at(t, i, j);
-      const T dz = dzy(P, o);
-      const T cy = celerity(c6(hr[0], hr[1], hr[2], hr[3], hr[4], hr[5]), dz);
-      const T vc = vpre + T(4.9) / cy * (hn - hs);
-      G = dz * vc + (vc > T(0) ? hn : hs) * vc;
-    }
-    s_G[q] = G;
-  }
-  __syncthreads();
 
-  // divergence and height update (agrid.py:44-53, barotr_pure.py:120)
-  for (int q = threadIdx.x; q < TI * TJ; q += kThreads) {
-    const int k = q / TJ, c = q - k * TJ;
-    if (k >= ni || c >= nj) continue;
-    const size_t o = g.at(t, i0 + k, j0 + c);
-    const T D = P.rda[o] * (((s_F[(k + 1) * TJ + c] * P.cdy[o + N] - s_F[k * TJ + c] * P.cdy[o]) +
-                             s_G[k * FJ + c + 1] * P.ddx[o + 1]) -
-                            s_G[k * FJ + c] * P.ddx[o]);
-    a.h_out[o] = a.hp[o] - D * a.dt;
+    // mass flux through the i-face at row i, column j
+    T F = T(0);
+    if (j <= N - 4) {
+      const bool edge = (i == HALO) || (i == N - HALO);
+      const int eidx = (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO);
+      if (a.last && edge) {
+        F = a.eflux[eidx];
+      } else {
+        const T dz = P.dz_uniform ? P.dzx_s : P.dzx[w.fo + static_cast<size_t>(i) * NS + jl];
+        F = face_flux(wh, wu, edge, edge ? a.evel[eidx] : T(0), dz);
+      }
+    }
+
+    if (k >= 1) {
+      const int ii = i - 1;
+      // row ii of vct and h into the row buffer
+      bv[lane + 3] = wvc[2];
+      bh[lane + 3] = wh[2];
+      if (halo_lane) {
+        bv[hslot] = ct_v(hr);
+        bh[hslot] = hh;
+      }
+      wave_lds_sync();
+      T G = T(0);
+      if (j <= N - 3) {
+        T sv[6], sh[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          sv[m] = bv[lane + m];
+          sh[m] = bh[lane + m];
+        }
+        const bool edge = (j == HALO) || (j == N - HALO);
+        const int eidx = (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO);
+        if (a.last && edge) {
+          G = a.eflux[eidx];
+        } else {
+          G = face_flux(sh, sv, edge, edge ? a.evel[eidx] : T(0), dzy_c);
+        }
+      }
+      wave_lds_sync();
+      const T Gn = __shfl_down(G, 1);
+      const T ddxn = __shfl_down(ddx_c, 1);
+      if (lane < kStripOut && j <= N - 4) {
+        // divergence (agrid.py:44-53) and height update (barotr_pure.py:120)
+        const T D = rda_c * (((F * cdy_i - Fprev * cdy_prev) + Gn * ddxn) - G * ddx_c);
+        a.h_out[w.fo + static_cast<size_t>(ii) * NS + j] = hp_c - D * a.dt;
+      }
+    }
+    Fprev = F;
+    cdy_prev = cdy_i;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      wu[m] = wu[m + 1];
+      wvc[m] = wvc[m + 1];
+      wh[m] = wh[m + 1];
+    }
   }
 }
 
-// New covariant velocity on the interior of one TI x TJ tile.
-template <typename T, int TI, int TJ>
-__global__ __launch_bounds__(kThreads) void k_stage_b(StageArgs<T> a) {
-  constexpr int HR = TI + 6, HC = TJ + 6, FJ = TJ + 1;
-  __shared__ T s_h[HR * HC];    // stage-input height (rect; corners unused)
-  __shared__ T s_n[HR * HC];    // blended new height (1-beta) hn + beta h
-  __shared__ T s_uctV[HR * TJ]; // contravariant u along i
-  __shared__ T s_vV[HR * TJ];   // covariant v along i
-  __shared__ T s_vctH[TI * HC]; // contravariant v along j
-  __shared__ T s_uH[TI * HC];   // covariant u along j
-  __shared__ T s_ucx[(TI + 1) * TJ], s_vcx[(TI + 1) * TJ], s_Cx[(TI + 1) * TJ];
-  __shared__ T s_vcy[TI * FJ], s_ucy[TI * FJ], s_Cy[TI * FJ];
-
-  const Geo g = a.g;
-  const int N = g.N, nx = g.nx;
-  int t, i0, j0;
-  tile_origin<TI, TJ>(a.nti, a.ntj, t, i0, j0);
-  const int ni = min(TI, N - HALO - i0), nj = min(TJ, N - HALO - j0);
+// Kernel B: new covariant velocity on the interior (barotr_pure.py:133-150).
+template <typename T>
+__global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
+  __shared__ T s_vct[4][kRowBuf];
+  __shared__ T s_u[4][kRowBuf];
+  __shared__ T s_h[4][kRowBuf];
+  __shared__ T s_n[4][kRowBuf];
+  WaveTile w;
+  if (!decode_wave(a, w)) return;
+  const int N = a.g.N, nx = a.g.nx;
   const Planes<T>& P = a.P;
+  T* bvct = s_vct[w.wslot];
+  T* bu = s_u[w.wslot];
+  T* bh = s_h[w.wslot];
+  T* bn = s_n[w.wslot];
+  const int lane = w.lane, j = w.j, jl = w.jl, c0 = w.c0, t = w.t;
+  const size_t NS = static_cast<size_t>(N);
   const T wn = T(1.0) - a.beta, wo = a.beta;
 
-  for (int q = threadIdx.x; q < HR * HC; q += kThreads) {
-    const int r = q / HC, c = q - r * HC;
-    if (r < ni + 6 && c < nj + 6) {
-      const size_t o = g.at(t, i0 - 3 + r, j0 - 3 + c);
-      const T ho = a.h[o];
-      s_h[q] = ho;
-      s_n[q] = wn * a.hn[o] + wo * ho;
-    }
-  }
-  for (int q = threadIdx.x; q < HR * TJ; q += kThreads) {
-    const int r = q / TJ, c = q - r * TJ;
-    if (r < ni + 6 && c < nj) {
-      const size_t o = g.at(t, i0 - 3 + r, j0 + c);
-      s_uctV[q] = uct_at(P, a.u, a.v, o);
-      s_vV[q] = a.v[o];
-    }
-  }
-  for (int q = threadIdx.x; q < TI * HC; q += kThreads) {
-    const int r = q / HC, c = q - r * HC;
-    if (r < ni && c < nj + 6) {
-      const size_t o = g.at(t, i0 + r, j0 - 3 + c);
-      s_vctH[q] = vct_at(P, a.u, a.v, o);
-      s_uH[q] = a.u[o];
-    }
-  }
-  __syncthreads();
+  int hcol = lane < 3 ? c0 - 3 + lane : c0 + 61 + lane;
+  hcol = min(hcol, N - 1);
+  const bool halo_lane = lane < 5;
+  const int hslot = hcol - c0 + 3;
 
-  // C-grid quantities on the i-faces
-  for (int q = threadIdx.x; q < (TI + 1) * TJ; q += kThreads) {
-    const int k = q / TJ, c = q - k * TJ;
-    if (k > ni || c >= nj) continue;
-    const int i = i0 + k, j = j0 + c;
-    const bool edge = (i == HALO) || (i == N - HALO);
-    const T* uc = &s_uctV[k * TJ + c];
-    const T ue = e5(uc[0], uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ]);
-    const T uw = w5(uc[TJ], uc[2 * TJ], uc[3 * TJ], uc[4 * TJ], uc[5 * TJ]);
-    const T upre = edge ? a.evel[(t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO)] : T(0.5) * (ue + uw);
-    const T* vv = &s_vV[k * TJ + c];
-    const T vcx = upre > T(0) ? e5(vv[0], vv[TJ], vv[2 * TJ], vv[3 * TJ], vv[4 * TJ])
-                              : w5(vv[TJ], vv[2 * TJ], vv[3 * TJ], vv[4 * TJ], vv[5 * TJ]);
-    const T* hc = &s_h[k * HC + c + 3];
-    const T he = e5(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC]);
-    const T hw = w5(hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]);
-    const size_t o = g.at(t, i, j);
-    const T cx = celerity(c6(hc[0], hc[HC], hc[2 * HC], hc[3 * HC], hc[4 * HC], hc[5 * HC]), dzx(P, o));
-    s_ucx[q] = upre + T(4.9) / cx * (he - hw);
-    s_vcx[q] = vcx;
-    s_Cx[q] = T(0.5) * cx * (ue - uw);
+  // windows over rows i-3 .. i+2 (slot 5 newest); blended new height over
+  // rows i-4 .. i+2
+  T wu[6], wvc[6], wv[6], wuu[6], wh[6], wn7[7];
+  for (int m = 0; m < 5; ++m) {
+    const size_t o = w.fo + static_cast<size_t>(w.r0 - 3 + m) * NS + jl;
+    CtRaw<T> r;
+    load_ct(a, o, r);
+    wu[m] = ct_u(r);
+    wvc[m] = ct_v(r);
+    wv[m] = r.v;
+    wuu[m] = r.u;
+    const T ho = a.h[o];
+    wh[m] = ho;
+    wn7[m + 1] = wn * a.hn[o] + wo * ho;
   }
-  // C-grid quantities on the j-faces
-  for (int q = threadIdx.x; q < TI * FJ; q += kThreads) {
-    const int k = q / FJ, c = q - k * FJ;
-    if (k >= ni || c > nj) continue;
-    const int i = i0 + k, j = j0 + c;
-    const bool edge = (j == HALO) || (j == N - HALO);
-    const T* vc = &s_vctH[k * HC + c];
-    const T vn = e5(vc[0], vc[1], vc[2], vc[3], vc[4]);
-    const T vs = w5(vc[1], vc[2], vc[3], vc[4], vc[5]);
-    const T vpre = edge ? a.evel[(t * 4 + (j == HALO ? DW : DE)) * nx + (i - HALO)] : T(0.5) * (vn + vs);
-    const T* uu = &s_uH[k * HC + c];
-    const T ucy = vpre > T(0) ? e5(uu[0], uu[1], uu[2], uu[3], uu[4]) : w5(uu[1], uu[2], uu[3], uu[4], uu[5]);
-    const T* hr = &s_h[(k + 3) * HC + c];
-    const T hn = e5(hr[0], hr[1], hr[2], hr[3], hr[4]);
-    const T hs = w5(hr[1], hr[2], hr[3], hr[4], hr[5]);
-    const size_t o = g.at(t, i, j);
-    const T cy = celerity(c6(hr[0], hr[1], hr[2], hr[3], hr[4], hr[5]), dzy(P, o));
-    s_vcy[q] = vpre + T(4.9) / cy * (hn - hs);
-    s_ucy[q] = ucy;
-    s_Cy[q] = T(0.5) * cy * (vn - vs);
+  wn7[0] = T(0);
+  CtRaw<T> nr;
+  T nh, nhn;
+  {
+    const size_t o = w.fo + static_cast<size_t>(w.r0 + 2) * NS + jl;
+    load_ct(a, o, nr);
+    nh = a.h[o];
+    nhn = a.hn[o];
   }
-  __syncthreads();
+  T n_cdy = P.cdy[w.mo + static_cast<size_t>(w.r0) * NS + jl];
+  T n_dzx = P.dzx_s;
+  if (!P.dz_uniform) n_dzx = P.dzx[w.fo + static_cast<size_t>(w.r0) * NS + jl];
+  T n_rda = T(0), n_ddx = T(0), n_rdx = T(0), n_rdy = T(0), n_af = T(0), n_up = T(0), n_vp = T(0);
+  T n_dzy = P.dzy_s;
+  CtRaw<T> nhr;
+  T nhh = T(0), nhn_h = T(0);
+  T ucx_prev = T(0), vcx_prev = T(0), Cx_prev = T(0), cdy_prev = T(0);
 
-  for (int q = threadIdx.x; q < TI * TJ; q += kThreads) {
-    const int k = q / TJ, c = q - k * TJ;
-    if (k >= ni || c >= nj) continue;
-    const int i = i0 + k, j = j0 + c;
-    const size_t o = g.at(t, i, j);
-    const int x0 = k * TJ + c, x1 = (k + 1) * TJ + c;  // i-face rows k, k+1
-    const int y0 = k * FJ + c, y1 = k * FJ + c + 1;    // j-face cols c, c+1
-    const T rdx = P.rdx[o], rdy = P.rdy[o];
-    // vorticity (agrid.py:32-41) with vv = vb_cx, uu = ub_cy
-    const T zeta = P.rda[o] * (((s_ucy[y0] * P.ddx[o] + s_vcx[x1] * P.cdy[o + N]) - s_ucy[y1] * P.ddx[o + 1]) -
-                               s_vcx[x0] * P.cdy[o]);
-    const T ku1 = T(0.5) * (s_ucx[x1] * s_ucx[x1] + s_vcx[x1] * s_vcx[x1]);
-    const T ku0 = T(0.5) * (s_ucx[x0] * s_ucx[x0] + s_vcx[x0] * s_vcx[x0]);
-    const T kv1 = T(0.5) * (s_vcy[y1] * s_vcy[y1] + s_ucy[y1] * s_ucy[y1]);
-    const T kv0 = T(0.5) * (s_vcy[y0] * s_vcy[y0] + s_ucy[y0] * s_ucy[y0]);
-    const T vct = s_vctH[k * HC + c + 3];
-    const T uct = s_uctV[(k + 3) * TJ + c];
-    const T advx = zeta * vct - (ku1 - ku0) * rdx;
-    const T advy = -zeta * uct - (kv1 - kv0) * rdy;
-    const T visx = -(rdx * (s_Cx[x1] - s_Cx[x0]));
-    const T visy = -(rdy * (s_Cy[y1] - s_Cy[y0]));
-    const T* nc = &s_n[k * HC + c + 3];  // column j, rows i-3 .. i+3
-    const T hx0 = c6(nc[0], nc[HC], nc[2 * HC], nc[3 * HC], nc[4 * HC], nc[5 * HC]);
-    const T hx1 = c6(nc[HC], nc[2 * HC], nc[3 * HC], nc[4 * HC], nc[5 * HC], nc[6 * HC]);
-    const T* nr = &s_n[(k + 3) * HC + c];  // row i, cols j-3 .. j+3
-    const T hy0 = c6(nr[0], nr[1], nr[2], nr[3], nr[4], nr[5]);
-    const T hy1 = c6(nr[1], nr[2], nr[3], nr[4], nr[5], nr[6]);
-    const T pgfu = T(-9.80) * (rdx * (hx1 - hx0));
-    const T pgfv = T(-9.80) * (rdy * (hy1 - hy0));
-    const T af = P.af[o];
-    a.u_out[o] = a.up[o] + (((pgfu + advx) + af * vct) + visx) * a.dt;
-    a.v_out[o] = a.vp[o] + (((pgfv + advy) - af * uct) + visy) * a.dt;
+  for (int k = 0; k <= w.r1 - w.r0; ++k) {
+    const int i = w.r0 + k;
+    wu[5] = ct_u(nr);
+    wvc[5] = ct_v(nr);
+    wv[5] = nr.v;
+    wuu[5] = nr.u;
+    wh[5] = nh;
+    wn7[6] = wn * nhn + wo * nh;
+    const T cdy_i = n_cdy, dzx_i = n_dzx;
+    const T rda_c = n_rda, ddx_c = n_ddx, rdx_c = n_rdx, rdy_c = n_rdy, af_c = n_af;
+    const T up_c = n_up, vp_c = n_vp, dzy_c = n_dzy;
+    const CtRaw<T> hr = nhr;
+    const T hh = nhh, hhn = nhn_h;
+    {
+      const int row = min(i + 3, N - 1);
+      const size_t o = w.fo + static_cast<size_t>(row) * NS + jl;
+      load_ct(a, o, nr);
+      nh = a.h[o];
+      nhn = a.hn[o];
+      const int mrow = min(i + 1, N - 1);
+      const size_t om = static_cast<size_t>(mrow) * NS + jl;
+      n_cdy = P.cdy[w.mo + om];
+      if (!P.dz_uniform) n_dzx = P.dzx[w.fo + om];
+      const size_t oc = static_cast<size_t>(i) * NS + jl;
+      n_rda = P.rda[w.mo + oc];
+      n_ddx = P.ddx[w.mo + oc];
+      n_rdx = P.rdx[w.mo + oc];
+      n_rdy = P.rdy[w.mo + oc];
+      n_af = P.af[w.fo + oc];
+      n_up = a.up[w.fo + oc];
+      n_vp = a.vp[w.fo + oc];
+      if (!P.dz_uniform) n_dzy = P.dzy[w.fo + oc];
+      if (halo_lane) {
+        const size_t oh = w.fo + static_cast<size_t>(i) * NS + hcol;
+        load_ct(a, oh, nhr);
+        nhh = a.h[oh];
+        nhn_h = a.hn[oh];
+      }
+    }
+
+    // C-grid quantities on the i-face at row i (remap.py:42-95, barotr_pure.py:29-34, 63-75)
+    T ucx = T(0), vcx = T(0), Cx = T(0);
+    {
+      const T ue = e5(wu[0], wu[1], wu[2], wu[3], wu[4]);
+      const T uw = w5(wu[1], wu[2], wu[3], wu[4], wu[5]);
+      const bool edge = ((i == HALO) || (i == N - HALO)) && j <= N - 4;
+      const T upre = edge ? a.evel[(t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO)] : T(0.5) * (ue + uw);
+      vcx = upre > T(0) ? e5(wv[0], wv[1], wv[2], wv[3], wv[4]) : w5(wv[1], wv[2], wv[3], wv[4], wv[5]);
+      const T he = e5(wh[0], wh[1], wh[2], wh[3], wh[4]);
+      const T hw = w5(wh[1], wh[2], wh[3], wh[4], wh[5]);
+      const T cx = celerity(c6(wh[0], wh[1], wh[2], wh[3], wh[4], wh[5]), dzx_i);
+      ucx = upre + T(4.9) / cx * (he - hw);
+      Cx = T(0.5) * cx * (ue - uw);
+    }
+
+    if (k >= 1) {
+      const int ii = i - 1;
+      bvct[lane + 3] = wvc[2];
+      bu[lane + 3] = wuu[2];
+      bh[lane + 3] = wh[2];
+      bn[lane + 3] = wn7[3];
+      if (halo_lane) {
+        bvct[hslot] = ct_v(hr);
+        bu[hslot] = hr.u;
+        bh[hslot] = hh;
+        bn[hslot] = wn * hhn + wo * hh;
+      }
+      wave_lds_sync();
+      T sv[6], su[6], sh[6], sn[7];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        sv[m] = bvct[lane + m];
+        su[m] = bu[lane + m];
+        sh[m] = bh[lane + m];
+        sn[m] = bn[lane + m];
+      }
+      sn[6] = bn[min(lane + 6, kRowBuf - 1)];
+      wave_lds_sync();
+      // C-grid quantities on the j-face at column j, row ii
+      T vcy = T(0), ucy = T(0), Cy = T(0);
+      {
+        const T vn = e5(sv[0], sv[1], sv[2], sv[3], sv[4]);
+        const T vs = w5(sv[1], sv[2], sv[3], sv[4], sv[5]);
+        const bool edge = (j == HALO) || (j == N - HALO);
+        const T vpre = edge ? a.evel[(t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO)] : T(0.5) * (vn + vs);
+        ucy = vpre > T(0) ? e5(su[0], su[1], su[2], su[3], su[4]) : w5(su[1], su[2], su[3], su[4], su[5]);
+        const T hn = e5(sh[0], sh[1], sh[2], sh[3], sh[4]);
+        const T hs = w5(sh[1], sh[2], sh[3], sh[4], sh[5]);
+        const T cy = celerity(c6(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]), dzy_c);
+        vcy = vpre + T(4.9) / cy * (hn - hs);
+        Cy = T(0.5) * cy * (vn - vs);
+      }
+      const T ucy_n = __shfl_down(ucy, 1);
+      const T vcy_n = __shfl_down(vcy, 1);
+      const T Cy_n = __shfl_down(Cy, 1);
+      const T ddx_n = __shfl_down(ddx_c, 1);
+      if (lane < kStripOut && j <= N - 4) {
+        // vorticity (agrid.py:32-41) with vv = vb_cx, uu = ub_cy
+        const T zeta = rda_c * (((ucy * ddx_c + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
+        const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
+        const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
+        const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
+        const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+        const T vct_c = wvc[2];
+        const T uct_c = wu[2];
+        const T advx = zeta * vct_c - (ku1 - ku0) * rdx_c;
+        const T advy = -zeta * uct_c - (kv1 - kv0) * rdy_c;
+        const T visx = -(rdx_c * (Cx - Cx_prev));
+        const T visy = -(rdy_c * (Cy_n - Cy));
+        const T hx0 = c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]);
+        const T hx1 = c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
+        const T hy0 = c6(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]);
+        const T hy1 = c6(sn[1], sn[2], sn[3], sn[4], sn[5], sn[6]);
+        const T pgfu = T(-9.80) * (rdx_c * (hx1 - hx0));
+        const T pgfv = T(-9.80) * (rdy_c * (hy1 - hy0));
+        const size_t oc = w.fo + static_cast<size_t>(ii) * NS + j;
+        a.u_out[oc] = up_c + (((pgfu + advx) + af_c * vct_c) + visx) * a.dt;
+        a.v_out[oc] = vp_c + (((pgfv + advy) - af_c * uct_c) + visy) * a.dt;
+      }
+    }
+    ucx_prev = ucx;
+    vcx_prev = vcx;
+    Cx_prev = Cx;
+    cdy_prev = cdy_i;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      wu[m] = wu[m + 1];
+      wvc[m] = wvc[m + 1];
+      wv[m] = wv[m + 1];
+      wuu[m] = wuu[m + 1];
+      wh[m] = wh[m + 1];
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) wn7[m] = wn7[m + 1];
   }
 }
 
@@ -904,16 +1078,29 @@ struct sw_ctx {
   size_t cells = 0;        // 6 * N * N
   void* planes = nullptr;  // kNumPlanes T planes
   int* locs = nullptr;     // loc_i, loc_j planes
-  void* scratch = nullptr; // 9 T fields: S1 (h,u,v), S2 (h,u,v), tmp (h,u,v)
+  void* scratch = nullptr; // 12 T fields: S1, S2 (stage scratch), X, Y (substep ping-pong)
   void* edges = nullptr;   // evel, eflux, edge-average buffer (24*nx each)
   double dzx_s = 0, dzy_s = 0;
   int dz_uniform = 0;
+  int shared = 0;          // rdx, rdy, rda, c_dy, d_dx identical on every face
+  int gsym = 0;            // a_gct symmetric with equal diagonal everywhere
+  int rows_a = 32;         // rows per wave of k_march_a
+  int rows_b = 32;         // rows per wave of k_march_b
+  hipStream_t capture = nullptr;  // private stream used only to record graphs
+  struct GraphEntry {
+    int rk, pairs;
+    double dtb;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;  // recorded substep-pair sequences
+  struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfRec>* prof = nullptr;  // set only by sw_profile
 };
 
 namespace {
-
-constexpr int TA_I = 16, TA_J = 64;  // k_stage_a tile
-constexpr int TB_I = 8, TB_J = 64;   // k_stage_b tile
 
 template <typename T>
 Planes<T> planes_of(const sw_ctx* c) {
@@ -932,8 +1119,27 @@ Planes<T> planes_of(const sw_ctx* c) {
   return P;
 }
 
+// Kernel classes timed by sw_profile.
+enum { K_EDGE = 0, K_MARCH_A = 1, K_HALO_H = 2, K_MARCH_B = 3, K_HALO_UV = 4, K_DIAG = 5, K_CLASSES = 6 };
+
+int prof_begin(const sw_ctx* c, int cls) {
+  if (!c->prof) return SW_OK;
+  sw_ctx::ProfRec r{cls, nullptr, nullptr};
+  SW_HIP(hipEventCreate(&r.a));
+  SW_HIP(hipEventCreate(&r.b));
+  SW_HIP(hipEventRecord(r.a, c->stream));
+  c->prof->push_back(r);
+  return SW_OK;
+}
+
+int prof_end(const sw_ctx* c) {
+  if (!c->prof) return SW_OK;
+  SW_HIP(hipEventRecord(c->prof->back().b, c->stream));
+  return SW_OK;
+}
+
 template <typename T>
-T* scratch_field(const sw_ctx* c, int k) {
+T* scratch_field(const sw_ctx* c, int k) {  // k in [0, 12)
   return static_cast<T*>(c->scratch) + static_cast<size_t>(k) * c->cells;
 }
 
@@ -972,33 +1178,47 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   T* evel = edge_buf<T>(c, 0);
   T* eflux = edge_buf<T>(c, 1);
   EdgeArgs<T> ea{g, route(), P, h, u, v, evel, eflux, last};
+  int rc = prof_begin(c, K_EDGE);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), dim3(256), 0,
                      c->stream, ea);
   SW_HIP(hipGetLastError());
+  if ((rc = prof_end(c))) return rc;
 
-  StageArgs<T> sa{};
-  sa.g = g;
-  sa.P = P;
-  sa.h = h; sa.hp = hp; sa.u = u; sa.v = v; sa.up = up; sa.vp = vp;
-  sa.hn = h_out;
-  sa.evel = evel; sa.eflux = eflux;
-  sa.h_out = h_out; sa.u_out = u_out; sa.v_out = v_out;
-  sa.dt = T(dt);
-  sa.beta = T(beta);
-  sa.last = last;
-  sa.nti = (c->nx + TA_I - 1) / TA_I;
-  sa.ntj = (c->nx + TA_J - 1) / TA_J;
-  hipLaunchKernelGGL((k_stage_a<T, TA_I, TA_J>), dim3(6 * sa.nti * sa.ntj), dim3(kThreads), 0,
-                     c->stream, sa);
+  MarchArgs<T> ma{};
+  ma.g = g;
+  ma.P = P;
+  ma.h = h; ma.hp = hp; ma.u = u; ma.v = v; ma.up = up; ma.vp = vp;
+  ma.hn = h_out;
+  ma.evel = evel; ma.eflux = eflux;
+  ma.h_out = h_out; ma.u_out = u_out; ma.v_out = v_out;
+  ma.dt = T(dt);
+  ma.beta = T(beta);
+  ma.last = last;
+  ma.S = (c->nx + kStripOut - 1) / kStripOut;
+  ma.shared = c->shared;
+  ma.gsym = c->gsym;
+  ma.R = c->rows_a;
+  ma.nbands = (c->nx + ma.R - 1) / ma.R;
+  ma.nwaves = 6 * ma.S * ma.nbands;
+  if ((rc = prof_begin(c, K_MARCH_A))) return rc;
+  hipLaunchKernelGGL(k_march_a<T>, dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
   SW_HIP(hipGetLastError());
-  int rc = launch_halo_scalar<T>(c, h_out, 1);
+  if ((rc = prof_end(c))) return rc;
+  if ((rc = prof_begin(c, K_HALO_H))) return rc;
+  rc = launch_halo_scalar<T>(c, h_out, 1);
   if (rc) return rc;
-  sa.nti = (c->nx + TB_I - 1) / TB_I;
-  sa.ntj = (c->nx + TB_J - 1) / TB_J;
-  hipLaunchKernelGGL((k_stage_b<T, TB_I, TB_J>), dim3(6 * sa.nti * sa.ntj), dim3(kThreads), 0,
-                     c->stream, sa);
+  if ((rc = prof_end(c))) return rc;
+  ma.R = c->rows_b;
+  ma.nbands = (c->nx + ma.R - 1) / ma.R;
+  ma.nwaves = 6 * ma.S * ma.nbands;
+  if ((rc = prof_begin(c, K_MARCH_B))) return rc;
+  hipLaunchKernelGGL(k_march_b<T>, dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
   SW_HIP(hipGetLastError());
-  return launch_halo_vector<T>(c, u_out, v_out);
+  if ((rc = prof_end(c))) return rc;
+  if ((rc = prof_begin(c, K_HALO_UV))) return rc;
+  if ((rc = launch_halo_vector<T>(c, u_out, v_out))) return rc;
+  return prof_end(c);
 }
 
 template <typename T>
@@ -1010,9 +1230,11 @@ int run_diag(const sw_ctx* c, const T* h, const T* u, const T* v, void* const ou
   d.evel = edge_buf<T>(c, 0);
   T** dst[10] = {&d.cx, &d.cy, &d.uct, &d.vct, &d.ucx, &d.ucy, &d.vcx, &d.vcy, &d.advx, &d.advy};
   for (int k = 0; k < 10; ++k) *dst[k] = static_cast<T*>(out[3 + k]);
+  int rc = prof_begin(c, K_DIAG);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_diag<T>, dim3(blocks_for(c->cells)), dim3(256), 0, c->stream, d);
   SW_HIP(hipGetLastError());
-  return SW_OK;
+  return prof_end(c);
 }
 
 int check_ptrs(const void* const* p, int n, const char* what) {
@@ -1033,7 +1255,72 @@ int check_disjoint(const sw_ctx* c, const void* const in[6], void* const out[13]
   return SW_OK;
 }
 
-// RK driver: nstage stages per substep with (dt fraction, beta, last).
+// One substep (RK2: 2 stages, RK3: 3 stages, barotr.py:47-59 / 70-85) from
+// (h, hp, u, v, up, vp) into dst.  Stage scratch S1 / S2 holds the inputs of
+// the last stage afterwards (the diagnostics are computed from them).
+template <typename T>
+int run_substep(const sw_ctx* c, int rk, double dtb, const T* h, const T* hp, const T* u,
+                const T* v, const T* up, const T* vp, T* const dst[3]) {
+  T* s1[3] = {scratch_field<T>(c, 0), scratch_field<T>(c, 1), scratch_field<T>(c, 2)};
+  T* s2[3] = {scratch_field<T>(c, 3), scratch_field<T>(c, 4), scratch_field<T>(c, 5)};
+  int rc;
+  if (rk == 2) {
+    if ((rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 2.0, 0.0, 0))) return rc;
+    return run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, dst[0], dst[1], dst[2], dtb, 0.0, 1);
+  }
+  if ((rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 3.0, 1.0, 0))) return rc;
+  if ((rc = run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, s2[0], s2[1], s2[2], dtb / 2.0, 1.0, 0)))
+    return rc;
+  return run_stage<T>(c, s2[0], hp, s2[1], s2[2], up, vp, dst[0], dst[1], dst[2], dtb, 1.0, 1);
+}
+
+constexpr int kGraphPairs = 16;  // substep pairs recorded in one graph
+
+// Graph executing `pairs` times the two substeps X -> Y -> X on the internal
+// ping-pong buffers; recorded once per (rk, dtb, pairs) on the private stream.
+template <typename T>
+int pair_graph(sw_ctx* c, int rk, double dtb, int pairs, hipGraphExec_t* exec) {
+  for (const auto& e : c->graphs)
+    if (e.rk == rk && e.pairs == pairs && e.dtb == dtb) {
+      *exec = e.exec;
+      return SW_OK;
+    }
+  if (!c->capture) SW_HIP(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
+  T* X[3] = {scratch_field<T>(c, 6), scratch_field<T>(c, 7), scratch_field<T>(c, 8)};
+  T* Y[3] = {scratch_field<T>(c, 9), scratch_field<T>(c, 10), scratch_field<T>(c, 11)};
+  hipStream_t user = c->stream;
+  c->stream = c->capture;
+  SW_HIP(hipStreamBeginCapture(c->capture, hipStreamCaptureModeThreadLocal));
+  int rc = SW_OK;
+  for (int p = 0; p < pairs && !rc; ++p) {
+    rc = run_substep<T>(c, rk, dtb, X[0], X[0], X[1], X[2], X[1], X[2], Y);
+    if (!rc) rc = run_substep<T>(c, rk, dtb, Y[0], Y[0], Y[1], Y[2], Y[1], Y[2], X);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(c->capture, &graph);
+  c->stream = user;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(SW_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(SW_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  if (c->graphs.size() >= 8) {
+    (void)hipGraphExecDestroy(c->graphs.front().exec);
+    c->graphs.erase(c->graphs.begin());
+  }
+  c->graphs.push_back({rk, pairs, dtb, x});
+  *exec = x;
+  return SW_OK;
+}
+
+// RK driver (barotr.py:41-87).  Eager: every substep is launched from the
+// host.  Graph mode (SW_GRAPH): the first substep reads the caller's fields
+// into the internal buffer X; the remaining ones replay recorded X -> Y -> X
+// pairs; the final h, u, v are copied to out.
 template <typename T>
 int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, double dtb, int flags,
            int rk) {
@@ -1044,6 +1331,7 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   rc = check_disjoint(c, in, out);
   if (rc) return rc;
   if (nbb < 0) return fail(SW_E_ARG, "nbb < 0");
+  if (!(dtb == dtb) || dtb > 1e300 || dtb < -1e300) return fail(SW_E_ARG, "dtb must be finite");
   const size_t bytes = c->cells * sizeof(T);
   if (nbb == 0) {
     SW_HIP(hipMemcpyAsync(out[0], in[0], bytes, hipMemcpyDeviceToDevice, c->stream));
@@ -1057,33 +1345,40 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   const T* v = static_cast<const T*>(in[3]);
   const T* up = static_cast<const T*>(in[4]);
   const T* vp = static_cast<const T*>(in[5]);
-  T* s1[3] = {scratch_field<T>(c, 0), scratch_field<T>(c, 1), scratch_field<T>(c, 2)};
-  T* s2[3] = {scratch_field<T>(c, 3), scratch_field<T>(c, 4), scratch_field<T>(c, 5)};
-  T* tmp[3] = {scratch_field<T>(c, 6), scratch_field<T>(c, 7), scratch_field<T>(c, 8)};
   T* fin[3] = {static_cast<T*>(out[0]), static_cast<T*>(out[1]), static_cast<T*>(out[2])};
-  const T* dh = nullptr;
-  const T* du = nullptr;
-  const T* dv = nullptr;
-  for (int k = 0; k < nbb; ++k) {
-    T** dst = ((nbb - 1 - k) % 2 == 0) ? fin : tmp;
-    if (rk == 2) {
-      rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 2.0, 0.0, 0);
-      if (rc) return rc;
-      rc = run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, dst[0], dst[1], dst[2], dtb, 0.0, 1);
-      if (rc) return rc;
-      dh = s1[0]; du = s1[1]; dv = s1[2];
-    } else {
-      rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 3.0, 1.0, 0);
-      if (rc) return rc;
-      rc = run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, s2[0], s2[1], s2[2], dtb / 2.0, 1.0, 0);
-      if (rc) return rc;
-      rc = run_stage<T>(c, s2[0], hp, s2[1], s2[2], up, vp, dst[0], dst[1], dst[2], dtb, 1.0, 1);
-      if (rc) return rc;
-      dh = s2[0]; du = s2[1]; dv = s2[2];
+  T* X[3] = {scratch_field<T>(c, 6), scratch_field<T>(c, 7), scratch_field<T>(c, 8)};
+  T* Y[3] = {scratch_field<T>(c, 9), scratch_field<T>(c, 10), scratch_field<T>(c, 11)};
+  const int last_stage_scratch = rk == 2 ? 0 : 3;
+  const T* dh = scratch_field<T>(c, last_stage_scratch);
+  const T* du = scratch_field<T>(c, last_stage_scratch + 1);
+  const T* dv = scratch_field<T>(c, last_stage_scratch + 2);
+
+  if ((flags & SW_GRAPH) && nbb >= 3 && !c->prof) {
+    if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, X))) return rc;
+    const int pairs = (nbb - 1) / 2;
+    int done = 0;
+    while (done < pairs) {
+      const int p = (pairs - done >= kGraphPairs) ? kGraphPairs : pairs - done;
+      hipGraphExec_t ex = nullptr;
+      if ((rc = pair_graph<T>(c, rk, dtb, p, &ex))) return rc;
+      SW_HIP(hipGraphLaunch(ex, c->stream));
+      done += p;
     }
-    h = hp = dst[0];
-    u = up = dst[1];
-    v = vp = dst[2];
+    T** cur = X;
+    if ((nbb - 1) % 2) {
+      if ((rc = run_substep<T>(c, rk, dtb, X[0], X[0], X[1], X[2], X[1], X[2], Y))) return rc;
+      cur = Y;
+    }
+    for (int k = 0; k < 3; ++k)
+      SW_HIP(hipMemcpyAsync(fin[k], cur[k], bytes, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    for (int k = 0; k < nbb; ++k) {
+      T** dst = ((nbb - 1 - k) % 2 == 0) ? fin : X;
+      if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, dst))) return rc;
+      h = hp = dst[0];
+      u = up = dst[1];
+      v = vp = dst[2];
+    }
   }
   if (!(flags & SW_SKIP_DIAG)) return run_diag<T>(c, dh, du, dv, out);
   return SW_OK;
@@ -1094,9 +1389,9 @@ int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
   const size_t n = c->cells;
   SW_HIP(hipMalloc(&c->planes, kNumPlanes * n * sizeof(T)));
   SW_HIP(hipMalloc(&c->locs, 2 * n * sizeof(int)));
-  SW_HIP(hipMalloc(&c->scratch, 9 * n * sizeof(T)));
+  SW_HIP(hipMalloc(&c->scratch, 12 * n * sizeof(T)));
   SW_HIP(hipMalloc(&c->edges, 3 * 24 * static_cast<size_t>(c->nx) * sizeof(T)));
-  SW_HIP(hipMemsetAsync(c->scratch, 0, 9 * n * sizeof(T), c->stream));
+  SW_HIP(hipMemsetAsync(c->scratch, 0, 12 * n * sizeof(T), c->stream));
   SW_HIP(hipMemsetAsync(c->edges, 0, 3 * 24 * static_cast<size_t>(c->nx) * sizeof(T), c->stream));
   T* p = static_cast<T*>(c->planes);
   RepackArgs<T> a{};
@@ -1157,6 +1452,25 @@ int validate_consts(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
   }
   bool uni = true;
   for (size_t o = 0; o < n && uni; ++o) uni = (dzx[o] == dzx[0]) && (dzy[o] == dzy[0]);
+  // Planes that are the same on every face can be read from face 0, and a
+  // symmetric metric with equal diagonal needs two of its four entries.
+  const size_t NN = static_cast<size_t>(N) * N;
+  std::vector<double> buf(n), gct(4 * n);
+  bool shared = true;
+  for (int k : {2, 3, 14, 15, 16}) {
+    SW_HIP(hipMemcpy(buf.data(), consts[k], n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int t = 1; t < 6 && shared; ++t)
+      shared = std::memcmp(buf.data(), buf.data() + t * NN, NN * sizeof(double)) == 0;
+  }
+  SW_HIP(hipMemcpy(gct.data(), consts[12], 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+  bool gsym = true;
+  for (size_t o = 0; o < n && gsym; ++o) {
+    uint64_t b[4];
+    std::memcpy(b, &gct[4 * o], sizeof(b));
+    gsym = (b[1] == b[2]) && (b[0] == b[3]);
+  }
+  c->shared = shared ? 1 : 0;
+  c->gsym = gsym ? 1 : 0;
   c->dz_uniform = uni ? 1 : 0;
   c->dzx_s = dzx[0];
   c->dzy_s = dzy[0];
@@ -1165,6 +1479,9 @@ int validate_consts(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
 
 void free_ctx(sw_ctx* c) {
   if (!c) return;
+  for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
+  c->graphs.clear();
+  if (c->capture) (void)hipStreamDestroy(c->capture);
   if (c->planes) (void)hipFree(c->planes);
   if (c->locs) (void)hipFree(c->locs);
   if (c->scratch) (void)hipFree(c->scratch);
@@ -1294,6 +1611,48 @@ int sw_routing_table(int32_t out[6 * 4 * 3]) {
   for (int t = 0; t < 6; ++t)
     for (int d = 0; d < 4; ++d)
       for (int k = 0; k < 3; ++k) out[(t * 4 + d) * 3 + k] = R.r[t][d][k];
+  return SW_OK;
+}
+
+int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb, double dtb,
+               double ms[6], int counts[6]) {
+  if (!ctx || !ms || !counts) return fail(SW_E_ARG, "null argument");
+  std::vector<sw_ctx::ProfRec> recs;
+  ctx->prof = &recs;
+  int rc = ctx->dtype == SW_F64 ? run_rk<double>(ctx, in, out, nbb, dtb, 0, 2)
+                                : run_rk<float>(ctx, in, out, nbb, dtb, 0, 2);
+  ctx->prof = nullptr;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  for (int k = 0; k < K_CLASSES; ++k) {
+    ms[k] = 0.0;
+    counts[k] = 0;
+  }
+  for (auto& r : recs) {
+    float t = 0.0f;
+    if (!rc && e == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      ms[r.cls] += t;
+      counts[r.cls] += 1;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(SW_E_HIP, std::string("sw_profile: ") + hipGetErrorString(e));
+  return SW_OK;
+}
+
+int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  if (rows_a < 4 || rows_a > 4096 || rows_b < 4 || rows_b > 4096)
+    return fail(SW_E_ARG, "rows per wave must be in [4, 4096]");
+  ctx->rows_a = rows_a;
+  ctx->rows_b = rows_b;
+  return SW_OK;
+}
+
+int sw_layout_flags(const sw_ctx* ctx, int* flags) {
+  if (!ctx || !flags) return fail(SW_E_ARG, "null argument");
+  *flags = (ctx->dz_uniform ? 1 : 0) | (ctx->shared ? 2 : 0) | (ctx->gsym ? 4 : 0);
   return SW_OK;
 }
 

@@ -163,3 +163,49 @@ def test_schedule_core_chunks():
     assert got[-1] is extra or np.array_equal(got[-1], extra)
     for k in range(23):
         _assert_exact(f"schedule {B.STATE_KEYS[k]}", _host(got[k]), want[k])
+
+
+def test_graph_replays_match_eager():
+    """Recorded substep-pair graphs replayed several times, with eager work and
+    host synchronisation between replays, give the eager results bit for bit."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(24)
+    ctx = barotr.SWContext(consts, torch.float64)
+    core = [_dev(x, torch) for x in state[:6]]
+    for nbb in (3, 7, 36, 37):
+        eager = ctx.rk(core, nbb, dt, graph=False)
+        for rep in range(3):
+            got = ctx.rk(core, nbb, dt, graph=True)
+            torch.cuda.synchronize()
+            _ = (core[0] * 2.0).sum().item()  # unrelated eager work between replays
+            for k in range(13):
+                _assert_exact(f"graph nbb={nbb} rep={rep} out{k}", _host(got[k]), _host(eager[k]))
+
+
+def test_profile_pass_matches_and_counts():
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(16)
+    ctx = barotr.SWContext(consts, torch.float64)
+    core = [_dev(x, torch) for x in state[:6]]
+    prof = ctx.profile(core, 4, dt)
+    assert prof["march_a"][1] == 8 and prof["march_b"][1] == 8 and prof["diag"][1] == 1
+    assert all(ms > 0 for ms, n in prof.values() if n)
+    flags = ctx.layout_flags()
+    assert flags == {"dz_uniform": True, "shared_metrics": True, "gct_symmetric": True}
+
+
+@pytest.mark.parametrize("rows", [(4, 4), (7, 13), (64, 100)])
+def test_band_height_does_not_change_results(rows):
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    nx = 70
+    _, consts = grid_consts(nx)
+    ctx = barotr.SWContext(consts, torch.float64)
+    core = random_core(nx, seed=3)
+    want = B.step_rk_logic(*core, consts, 20.0, 0.0, True)
+    ctx.set_tuning(*rows)
+    got = ctx.stage(*[_dev(x, torch) for x in core], 20.0, 0.0, True)
+    for name, gv, wv in zip(NAMES13, got, want):
+        _assert_exact(f"rows={rows} {name}", _host(gv), wv)
