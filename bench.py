@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--nbb", type=int, default=2000, help="substeps per step")
     p.add_argument("--dtype", choices=("f64", "f32"), default="f64")
     p.add_argument("--no-graph", action="store_true", help="launch every substep eagerly")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default band height instead of timing the candidates")
     p.add_argument("--profile-substeps", type=int, default=20,
                    help="substeps of the eager event-timed pass (0 disables)")
     p.add_argument("--cpu-substeps", type=int, default=2,
@@ -121,6 +123,11 @@ def main() -> int:
     consts = barotr.as_device_consts(duogrid.barotr_consts(g, 5000.0), dev)
     state = tuple(torch.as_tensor(x).to(device=dev, dtype=tdt) for x in initial.state_tuple(h, u, v))
     ctx = barotr.context_for(consts, tdt)
+    tuning = None
+    if not args.no_autotune:
+        from licom4py_gpu_amd import autotune
+        tuning = autotune.tune(ctx, state[:6], dt, use_cache=False)
+        tuning = {k: tuning[k] for k in ("rows_a", "rows_b", "table")}
     setup_s = time.perf_counter() - t_setup
 
     def step(s):
@@ -191,7 +198,7 @@ def main() -> int:
         "config": {"workload": f"C{nx} {args.dtype.upper()} RK2 barotropic step, {nbb} substeps per step"
                                + ("" if args.no_graph else ", hipGraph substep pairs"),
                    "nx": nx, "interior_cells": cells, "substeps_per_step": nbb,
-                   "dt_s": dt, "setup_s": round(setup_s, 2)},
+                   "dt_s": dt, "setup_s": round(setup_s, 2), "tuning": tuning},
         "roofline": roofline,
         "roofline_path": roofline_path,
         "kernels": kernels,

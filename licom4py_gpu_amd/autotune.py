@@ -1,0 +1,84 @@
+"""Project autotuner for the stage kernels' launch shape.
+
+Each wavefront of the two stage kernels marches down a band of R rows of a
+63-column strip; R trades parallelism (more, shorter bands fill more SIMDs)
+against the 6-row stencil warm-up every band repeats.  `tune` times a short
+eager run for every candidate (R_height, R_velocity) on the context's own
+resolution and precision, keeps the fastest, and records it in a JSON cache
+keyed by (nx, dtype, device name).  Results do not depend on R (the kernels
+produce identical bits for any band height), so tuning is free to change it.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Dict, Iterable, Optional, Sequence, Tuple
+
+CANDIDATES: Tuple[int, ...] = (8, 12, 16, 24, 32, 48)
+CACHE_ENV = "LICOM_SW_TUNE_CACHE"
+
+
+def _cache_path() -> str:
+    return os.environ.get(CACHE_ENV, os.path.join(os.path.expanduser("~"), ".cache", "licom_sw_tune.json"))
+
+
+def _key(nx: int, dtype, device_name: str) -> str:
+    return f"{device_name}|C{nx}|{str(dtype).replace('torch.', '')}"
+
+
+def load_cache() -> Dict[str, dict]:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(cache: Dict[str, dict]) -> None:
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path + ".tmp", "w") as f:
+            json.dump(cache, f, indent=1)
+        os.replace(path + ".tmp", path)
+    except OSError:
+        pass
+
+
+def tune(ctx, core6: Sequence, dt: float, substeps: int = 4, reps: int = 2,
+         candidates: Iterable[int] = CANDIDATES, use_cache: bool = True) -> dict:
+    """Pick (rows_a, rows_b) for `ctx` by timing; applies and returns it."""
+    import torch
+
+    dev_name = torch.cuda.get_device_name(ctx.device)
+    key = _key(ctx.nx, ctx.dtype, dev_name)
+    cache = load_cache() if use_cache else {}
+    if key in cache:
+        best = cache[key]
+        ctx.set_tuning(best["rows_a"], best["rows_b"])
+        return dict(best, cached=True)
+    cands = [c for c in candidates if c <= max(4, ctx.nx)]
+    times = {}
+    # tune the two kernels independently with the per-class event timer
+    for r in cands:
+        ctx.set_tuning(r, r)
+        ctx.profile(core6, 1, dt)  # warm
+        best_a = best_b = float("inf")
+        for _ in range(reps):
+            prof = ctx.profile(core6, substeps, dt)
+            best_a = min(best_a, prof["march_a"][0] / max(prof["march_a"][1], 1))
+            best_b = min(best_b, prof["march_b"][0] / max(prof["march_b"][1], 1))
+        times[r] = (best_a, best_b)
+    ra = min(cands, key=lambda r: times[r][0])
+    rb = min(cands, key=lambda r: times[r][1])
+    ctx.set_tuning(ra, rb)
+    result = {"rows_a": ra, "rows_b": rb,
+              "ms_a": times[ra][0], "ms_b": times[rb][1],
+              "table": {str(r): {"ms_a": t[0], "ms_b": t[1]} for r, t in times.items()},
+              "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+    if use_cache:
+        cache[key] = result
+        save_cache(cache)
+    return dict(result, cached=False)

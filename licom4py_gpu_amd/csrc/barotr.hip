@@ -306,6 +306,7 @@ struct MarchArgs {
   int S, R, nbands, nwaves;
   int shared;  // rdx, rdy, rda, c_dy, d_dx identical on every face
   int gsym;    // a_gct symmetric with equal diagonal
+  T* sink;     // 256 elements absorbing the stores of inactive lanes
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -314,9 +315,20 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Element i of a field addressed through a 32-bit byte offset from a uniform
+// base, so loads and stores use the scalar-base + 32-bit vector-offset form.
+template <typename T>
+__device__ __forceinline__ T ldg(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<uint32_t>(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ T* at32(T* base, uint32_t i) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<uint32_t>(sizeof(T)));
+}
+
 struct WaveTile {
   int t, lane, wslot, c0, j, jl, r0, r1;
-  size_t fo, mo;
+  uint32_t fo, mo;  // element offsets (fields stay below 2^29 elements)
 };
 
 template <typename T>
@@ -335,8 +347,8 @@ __device__ __forceinline__ bool decode_wave(const MarchArgs<T>& a, WaveTile& w) 
   w.jl = min(w.j, N - 1);
   w.r0 = HALO + band * a.R;
   w.r1 = min(w.r0 + a.R, N - HALO);
-  const size_t NN = static_cast<size_t>(N) * N;
-  w.fo = static_cast<size_t>(w.t) * NN;
+  const uint32_t NN = static_cast<uint32_t>(N) * N;
+  w.fo = static_cast<uint32_t>(w.t) * NN;
   w.mo = a.shared ? 0 : w.fo;
   return true;
 }
@@ -348,163 +360,181 @@ struct CtRaw {
 };
 
 template <typename T>
-__device__ __forceinline__ void load_ct(const MarchArgs<T>& a, size_t o, CtRaw<T>& r) {
-  const Planes<T>& P = a.P;
-  r.u = a.u[o];
-  r.v = a.v[o];
-  r.g00 = P.g00[o];
-  r.g01 = P.g01[o];
-  r.sina = P.sina[o];
-  if (a.gsym) {
-    r.g10 = r.g01;
-    r.g11 = r.g00;
-  } else {
-    r.g10 = P.g10[o];
-    r.g11 = P.g11[o];
-  }
-}
-template <typename T>
 __device__ __forceinline__ T ct_u(const CtRaw<T>& r) { return (r.g00 * r.u + r.g01 * r.v) * r.sina; }
 template <typename T>
 __device__ __forceinline__ T ct_v(const CtRaw<T>& r) { return (r.g10 * r.u + r.g11 * r.v) * r.sina; }
 
-// Mass flux through one face from its 6-point column of h and contravariant
-// velocity (barotr_pure.py:29-49; remap.py:42-55): s[] and w[] hold samples
-// k-3 .. k+2 across the face, upre_edge is used on a cube edge.
+// Kernel A: new height on the interior (barotr_pure.py:111-120).
+//
+// Every memory operation in the row loop is unconditional (addresses of
+// inactive lanes are clamped, their stores go to a per-wave sink), so the
+// compiler can count outstanding loads exactly and the loads issued one row
+// ahead stay in flight across the row's arithmetic.
 template <typename T>
-__device__ __forceinline__ T face_flux(const T* s, const T* w, bool edge, T upre_edge, T dz) {
+struct APre {
+  CtRaw<T> cr;   // column point of row i+2
+  T h;
+  T cdy, dzx;    // i-face row i
+  T evi, efi;    // edge velocity / flux of the i-face (valid on cube edges)
+  T rda, ddx, hp, dzy;  // cell row i-1
+  T evj, efj;    // edge velocity / flux of the j-face at (i-1, j)
+  CtRaw<T> hr;   // row-buffer column outside the strip, row i-1
+  T hh;
+};
+
+template <typename T, bool GSYM>
+__device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRaw<T>& r) {
+  const Planes<T>& P = a.P;
+  r.u = ldg(a.u, o);
+  r.v = ldg(a.v, o);
+  r.g00 = ldg(P.g00, o);
+  r.g01 = ldg(P.g01, o);
+  r.sina = ldg(P.sina, o);
+  if (GSYM) {
+    r.g10 = r.g01;
+    r.g11 = r.g00;
+  } else {
+    r.g10 = ldg(P.g10, o);
+    r.g11 = ldg(P.g11, o);
+  }
+}
+
+template <typename T, bool GSYM>
+__device__ __forceinline__ void a_prefetch(const MarchArgs<T>& a, const WaveTile& w, int i, int hcol,
+                                           APre<T>& p) {
+  const int N = a.g.N, nx = a.g.nx;
+  const Planes<T>& P = a.P;
+  const uint32_t NS = static_cast<uint32_t>(N);
+  const int jl = w.jl, j = w.j, t = w.t;
+  const uint32_t oc = w.fo + static_cast<uint32_t>(min(i + 2, N - 1)) * NS + jl;
+  load_ct2<T, GSYM>(a, oc, p.cr);
+  p.h = ldg(a.h, oc);
+  const uint32_t oi = static_cast<uint32_t>(min(i, N - 1)) * NS + jl;
+  p.cdy = ldg(P.cdy, w.mo + oi);
+  p.dzx = ldg(P.dzx, P.dz_uniform ? 0 : w.fo + oi);
+  const bool iedge = (i == HALO || i == N - HALO) && j <= N - 4;
+  const int ie = iedge ? (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO) : 0;
+  p.evi = ldg(a.evel, ie);
+  p.efi = ldg(a.eflux, ie);
+  const int ii = max(i - 1, 0);
+  const uint32_t om = static_cast<uint32_t>(ii) * NS + jl;
+  p.rda = ldg(P.rda, w.mo + om);
+  p.ddx = ldg(P.ddx, w.mo + om);
+  p.hp = ldg(a.hp, w.fo + om);
+  p.dzy = ldg(P.dzy, P.dz_uniform ? 0 : w.fo + om);
+  const bool jedge = (j == HALO || j == N - HALO) && ii >= HALO && ii < N - HALO;
+  const int je = jedge ? (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
+  p.evj = ldg(a.evel, je);
+  p.efj = ldg(a.eflux, je);
+  const uint32_t oh = w.fo + static_cast<uint32_t>(ii) * NS + hcol;
+  load_ct2<T, GSYM>(a, oh, p.hr);
+  p.hh = ldg(a.h, oh);
+}
+
+// Mass flux through one face (barotr_pure.py:29-49) given the averaged normal
+// velocity on cube edges; s[] = h and w[] = contravariant velocity samples
+// k-3 .. k+2 across the face.
+template <typename T>
+__device__ __forceinline__ T face_flux2(const T* s, const T* w, bool edge, T upre_edge, T dz) {
   const T he = e5(s[0], s[1], s[2], s[3], s[4]);
   const T hw = w5(s[1], s[2], s[3], s[4], s[5]);
-  const T upre = edge ? upre_edge
-                      : T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
+  const T inner = T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
+  const T upre = edge ? upre_edge : inner;
   const T c = celerity(c6(s[0], s[1], s[2], s[3], s[4], s[5]), dz);
   const T uc = upre + T(4.9) / c * (he - hw);
   return dz * uc + (uc > T(0) ? he : hw) * uc;
 }
 
-// Kernel A: new height on the interior (barotr_pure.py:111-120).
-template <typename T>
+template <typename T, bool GSYM>
 __global__ __launch_bounds__(256) void k_march_a(MarchArgs<T> a) {
   __shared__ T s_v[4][kRowBuf];
   __shared__ T s_h[4][kRowBuf];
   WaveTile w;
   if (!decode_wave(a, w)) return;
-  const int N = a.g.N, nx = a.g.nx;
-  const Planes<T>& P = a.P;
+  const int N = a.g.N;
   T* bv = s_v[w.wslot];
   T* bh = s_h[w.wslot];
-  const int lane = w.lane, j = w.j, jl = w.jl, c0 = w.c0, t = w.t;
-  const size_t NS = static_cast<size_t>(N);
+  const int lane = w.lane, j = w.j, c0 = w.c0;
+  const uint32_t NS = static_cast<uint32_t>(N);
 
-  // lanes 0..4 also supply the row-buffer columns outside the strip
   int hcol = lane < 3 ? c0 - 3 + lane : c0 + 61 + lane;
-  hcol = min(hcol, N - 1);
   const bool halo_lane = lane < 5;
-  const int hslot = hcol - c0 + 3;
+  hcol = halo_lane ? min(hcol, N - 1) : w.jl;
+  const int hslot = halo_lane ? hcol - c0 + 3 : lane + 3;  // others rewrite their own slot
+  const bool out_lane = lane < kStripOut && j <= N - 4;
+  T* sink = a.sink + (blockIdx.x * 256 + threadIdx.x) % 256;
 
   T wu[6], wvc[6], wh[6];
   for (int m = 0; m < 5; ++m) {
-    const size_t o = w.fo + static_cast<size_t>(w.r0 - 3 + m) * NS + jl;
+    const uint32_t o = w.fo + static_cast<uint32_t>(w.r0 - 3 + m) * NS + w.jl;
     CtRaw<T> r;
-    load_ct(a, o, r);
+    load_ct2<T, GSYM>(a, o, r);
     wu[m] = ct_u(r);
     wvc[m] = ct_v(r);
-    wh[m] = a.h[o];
+    wh[m] = ldg(a.h, o);
   }
-  // prefetch registers: column point of row i+2, metrics of row i / cell row i-1
-  CtRaw<T> nr;
-  T nh;
+  APre<T> pre;
+  a_prefetch<T, GSYM>(a, w, w.r0, hcol, pre);
+
+  // step 0: the i-face at row r0 only
+  T Fprev, cdy_prev;
   {
-    const size_t o = w.fo + static_cast<size_t>(w.r0 + 2) * NS + jl;
-    load_ct(a, o, nr);
-    nh = a.h[o];
-  }
-  T n_cdy = P.cdy[w.mo + static_cast<size_t>(w.r0) * NS + jl];
-  T n_rda = T(0), n_ddx = T(0), n_hp = T(0), n_dzy = a.P.dzy_s;
-  CtRaw<T> nhr;
-  T nhh = T(0);
-  T Fprev = T(0), cdy_prev = T(0);
-
-  for (int k = 0; k <= w.r1 - w.r0; ++k) {
-    const int i = w.r0 + k;
-    // consume the prefetched row i+2
-    wu[5] = ct_u(nr);
-    wvc[5] = ct_v(nr);
-    wh[5] = nh;
-    const T cdy_i = n_cdy;
-    const T rda_c = n_rda, ddx_c = n_ddx, hp_c = n_hp, dzy_c = n_dzy;
-    const CtRaw<T> hr = nhr;
-    const T hh = nhh;
-    // issue the loads of the next step
-    {
-      const int row = min(i + 3, N - 1);
-      const size_t o = w.fo + static_cast<size_t>(row) * NS + jl;
-      load_ct(a, o, nr);
-      nh = a.h[o];
-      const int mrow = min(i + 1, N - 1);
-      n_cdy = P.cdy[w.mo + static_cast<size_t>(mrow) * NS + jl];
-      const size_t oc = static_cast<size_t>(i) * NS + jl;
-      n_rda = P.rda[w.mo + oc];
-      n_ddx = P.ddx[w.mo + oc];
-      n_hp = a.hp[w.fo + oc];
-      if (!P.dz_uniform) n_dzy = P.dzy[w.fo + oc];
-      if (halo_lane) {
-        const size_t oh = w.fo + static_cast<size_t>(i) * NS + hcol;
-        load_ct(a, oh, nhr);
-        nhh = a.h[oh];
-      }
-    }
-
-    // mass flux through the i-face at row i, column j
-    T F = T(0);
-    if (j <= N - 4) {
-      const bool edge = (i == HALO) || (i == N - HALO);
-      const int eidx = (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO);
-      if (a.last && edge) {
-        F = a.eflux[eidx];
-      } else {
-        const T dz = P.dz_uniform ? P.dzx_s : P.dzx[w.fo + static_cast<size_t>(i) * NS + jl];
-        F = face_flux(wh, wu, edge, edge ? a.evel[eidx] : T(0), dz);
-      }
-    }
-
-    if (k >= 1) {
-      const int ii = i - 1;
-      // row ii of vct and h into the row buffer
-      bv[lane + 3] = wvc[2];
-      bh[lane + 3] = wh[2];
-      if (halo_lane) {
-        bv[hslot] = ct_v(hr);
-        bh[hslot] = hh;
-      }
-      wave_lds_sync();
-      T G = T(0);
-      if (j <= N - 3) {
-        T sv[6], sh[6];
+    const APre<T> cur = pre;
+    a_prefetch<T, GSYM>(a, w, w.r0 + 1, hcol, pre);
+    wu[5] = ct_u(cur.cr);
+    wvc[5] = ct_v(cur.cr);
+    wh[5] = cur.h;
+    const int i = w.r0;
+    const bool edge = (i == HALO) || (i == N - HALO);
+    const T F = face_flux2(wh, wu, edge, cur.evi, cur.dzx);
+    Fprev = (a.last && edge) ? cur.efi : F;
+    cdy_prev = cur.cdy;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-          sv[m] = bv[lane + m];
-          sh[m] = bh[lane + m];
-        }
-        const bool edge = (j == HALO) || (j == N - HALO);
-        const int eidx = (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO);
-        if (a.last && edge) {
-          G = a.eflux[eidx];
-        } else {
-          G = face_flux(sh, sv, edge, edge ? a.evel[eidx] : T(0), dzy_c);
-        }
-      }
-      wave_lds_sync();
-      const T Gn = __shfl_down(G, 1);
-      const T ddxn = __shfl_down(ddx_c, 1);
-      if (lane < kStripOut && j <= N - 4) {
-        // divergence (agrid.py:44-53) and height update (barotr_pure.py:120)
-        const T D = rda_c * (((F * cdy_i - Fprev * cdy_prev) + Gn * ddxn) - G * ddx_c);
-        a.h_out[w.fo + static_cast<size_t>(ii) * NS + j] = hp_c - D * a.dt;
-      }
+    for (int m = 0; m < 5; ++m) {
+      wu[m] = wu[m + 1];
+      wvc[m] = wvc[m + 1];
+      wh[m] = wh[m + 1];
     }
+  }
+
+  for (int i = w.r0 + 1; i <= w.r1; ++i) {
+    const APre<T> cur = pre;
+    a_prefetch<T, GSYM>(a, w, i + 1, hcol, pre);
+    wu[5] = ct_u(cur.cr);
+    wvc[5] = ct_v(cur.cr);
+    wh[5] = cur.h;
+
+    // mass flux through the i-face at row i
+    const bool iedge = (i == HALO) || (i == N - HALO);
+    T F = face_flux2(wh, wu, iedge, cur.evi, cur.dzx);
+    F = (a.last && iedge) ? cur.efi : F;
+
+    // row i-1 of vct and h through the row buffer
+    const int ii = i - 1;
+    bv[lane + 3] = wvc[2];
+    bh[lane + 3] = wh[2];
+    bv[hslot] = ct_v(cur.hr);
+    bh[hslot] = cur.hh;
+    wave_lds_sync();
+    T sv[6], sh[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      sv[m] = bv[lane + m];
+      sh[m] = bh[lane + m];
+    }
+    wave_lds_sync();
+    const bool jedge = (j == HALO) || (j == N - HALO);
+    T G = face_flux2(sh, sv, jedge, cur.evj, cur.dzy);
+    G = (a.last && jedge) ? cur.efj : G;
+    const T Gn = __shfl_down(G, 1);
+    const T ddxn = __shfl_down(cur.ddx, 1);
+    // divergence (agrid.py:44-53) and height update (barotr_pure.py:120)
+    const T D = cur.rda * (((F * cur.cdy - Fprev * cdy_prev) + Gn * ddxn) - G * cur.ddx);
+    T* dst = out_lane ? at32(a.h_out, w.fo + static_cast<uint32_t>(ii) * NS + j) : sink;
+    *dst = cur.hp - D * a.dt;
+
     Fprev = F;
-    cdy_prev = cdy_i;
+    cdy_prev = cur.cdy;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
       wu[m] = wu[m + 1];
@@ -515,7 +545,112 @@ __global__ __launch_bounds__(256) void k_march_a(MarchArgs<T> a) {
 }
 
 // Kernel B: new covariant velocity on the interior (barotr_pure.py:133-150).
+// The per-row inputs are fetched one row ahead in four groups, each issued
+// right after the current values of that group are consumed, so the current
+// and the next values of a group are never live together.
 template <typename T>
+struct BCol {  // column point of row i+2
+  T u, v, g00, g01, g10, g11, sina, h, hn;
+};
+template <typename T>
+struct BFace {  // i-face row i
+  T cdy, dzx, evi;
+};
+template <typename T>
+struct BHalo {  // row-buffer column outside the strip, row i-1
+  T u, v, g10, g11, sina, h, hn;
+};
+template <typename T>
+struct BCell {  // cell row i-1
+  T rda, ddx, rdx, rdy, af, up, vp, dzy, evj;
+};
+
+template <typename T, bool GSYM>
+__device__ __forceinline__ void b_fetch_col(const MarchArgs<T>& a, const WaveTile& w, int i, BCol<T>& p) {
+  const int N = a.g.N;
+  const Planes<T>& P = a.P;
+  const uint32_t o = w.fo + static_cast<uint32_t>(min(i + 2, N - 1)) * N + w.jl;
+  p.u = ldg(a.u, o);
+  p.v = ldg(a.v, o);
+  p.g00 = ldg(P.g00, o);
+  p.g01 = ldg(P.g01, o);
+  if (GSYM) {
+    p.g10 = p.g01;
+    p.g11 = p.g00;
+  } else {
+    p.g10 = ldg(P.g10, o);
+    p.g11 = ldg(P.g11, o);
+  }
+  p.sina = ldg(P.sina, o);
+  p.h = ldg(a.h, o);
+  p.hn = ldg(a.hn, o);
+}
+template <typename T>
+__device__ __forceinline__ void b_fetch_face(const MarchArgs<T>& a, const WaveTile& w, int i, BFace<T>& p) {
+  const int N = a.g.N, nx = a.g.nx;
+  const Planes<T>& P = a.P;
+  const uint32_t oi = static_cast<uint32_t>(min(i, N - 1)) * N + w.jl;
+  p.cdy = ldg(P.cdy, w.mo + oi);
+  p.dzx = ldg(P.dzx, P.dz_uniform ? 0 : w.fo + oi);
+  const bool iedge = (i == HALO || i == N - HALO) && w.j <= N - 4;
+  p.evi = ldg(a.evel, iedge ? (w.t * 4 + (i == HALO ? DS : DN)) * nx + (w.j - HALO) : 0);
+}
+template <typename T, bool GSYM>
+__device__ __forceinline__ void b_fetch_halo(const MarchArgs<T>& a, const WaveTile& w, int ii, int hcol,
+                                             BHalo<T>& p) {
+  const int N = a.g.N;
+  const Planes<T>& P = a.P;
+  const uint32_t o = w.fo + static_cast<uint32_t>(min(ii, N - 1)) * N + hcol;
+  p.u = ldg(a.u, o);
+  p.v = ldg(a.v, o);
+  if (GSYM) {
+    p.g10 = ldg(P.g01, o);
+    p.g11 = ldg(P.g00, o);
+  } else {
+    p.g10 = ldg(P.g10, o);
+    p.g11 = ldg(P.g11, o);
+  }
+  p.sina = ldg(P.sina, o);
+  p.h = ldg(a.h, o);
+  p.hn = ldg(a.hn, o);
+}
+template <typename T>
+__device__ __forceinline__ void b_fetch_cell(const MarchArgs<T>& a, const WaveTile& w, int ii, BCell<T>& p) {
+  const int N = a.g.N, nx = a.g.nx;
+  const Planes<T>& P = a.P;
+  const uint32_t om = static_cast<uint32_t>(min(ii, N - 1)) * N + w.jl;
+  p.rda = ldg(P.rda, w.mo + om);
+  p.ddx = ldg(P.ddx, w.mo + om);
+  p.rdx = ldg(P.rdx, w.mo + om);
+  p.rdy = ldg(P.rdy, w.mo + om);
+  p.af = ldg(P.af, w.fo + om);
+  p.up = ldg(a.up, w.fo + om);
+  p.vp = ldg(a.vp, w.fo + om);
+  p.dzy = ldg(P.dzy, P.dz_uniform ? 0 : w.fo + om);
+  const bool jedge = (w.j == HALO || w.j == N - HALO) && ii >= HALO && ii < N - HALO;
+  p.evj = ldg(a.evel, jedge ? (w.t * 4 + (w.j == HALO ? DW : DE)) * nx + (ii - HALO) : 0);
+}
+
+// C-grid quantities on the i-face at row i from the column windows
+// (remap.py:42-95, barotr_pure.py:29-34, 63-75).
+template <typename T>
+__device__ __forceinline__ void iface_b(const T* wu, const T* wv, const T* wh, bool edge, T evi, T dzx,
+                                        T& ucx, T& vcx, T& Cx) {
+  const T ue = e5(wu[0], wu[1], wu[2], wu[3], wu[4]);
+  const T uw = w5(wu[1], wu[2], wu[3], wu[4], wu[5]);
+  const T inner = T(0.5) * (ue + uw);
+  const T upre = edge ? evi : inner;
+  const T ve = e5(wv[0], wv[1], wv[2], wv[3], wv[4]);
+  const T vw = w5(wv[1], wv[2], wv[3], wv[4], wv[5]);
+  vcx = upre > T(0) ? ve : vw;
+  const T he = e5(wh[0], wh[1], wh[2], wh[3], wh[4]);
+  const T hw = w5(wh[1], wh[2], wh[3], wh[4], wh[5]);
+  const T cx = celerity(c6(wh[0], wh[1], wh[2], wh[3], wh[4], wh[5]), dzx);
+  ucx = upre + T(4.9) / cx * (he - hw);
+  Cx = T(0.5) * cx * (ue - uw);
+}
+
+template <typename T, bool GSYM>
 __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
   __shared__ T s_vct[4][kRowBuf];
   __shared__ T s_u[4][kRowBuf];
@@ -523,174 +658,156 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
   __shared__ T s_n[4][kRowBuf];
   WaveTile w;
   if (!decode_wave(a, w)) return;
-  const int N = a.g.N, nx = a.g.nx;
-  const Planes<T>& P = a.P;
+  const int N = a.g.N;
   T* bvct = s_vct[w.wslot];
   T* bu = s_u[w.wslot];
   T* bh = s_h[w.wslot];
   T* bn = s_n[w.wslot];
-  const int lane = w.lane, j = w.j, jl = w.jl, c0 = w.c0, t = w.t;
-  const size_t NS = static_cast<size_t>(N);
+  const int lane = w.lane, j = w.j, c0 = w.c0;
+  const uint32_t NS = static_cast<uint32_t>(N);
   const T wn = T(1.0) - a.beta, wo = a.beta;
 
   int hcol = lane < 3 ? c0 - 3 + lane : c0 + 61 + lane;
-  hcol = min(hcol, N - 1);
   const bool halo_lane = lane < 5;
-  const int hslot = hcol - c0 + 3;
+  hcol = halo_lane ? min(hcol, N - 1) : w.jl;
+  const int hslot = halo_lane ? hcol - c0 + 3 : lane + 3;  // others rewrite their own slot
+  const bool out_lane = lane < kStripOut && j <= N - 4;
+  T* sink = a.sink + (blockIdx.x * 256 + threadIdx.x) % 256;
 
-  // windows over rows i-3 .. i+2 (slot 5 newest); blended new height over
-  // rows i-4 .. i+2
-  T wu[6], wvc[6], wv[6], wuu[6], wh[6], wn7[7];
+  // column windows over rows i-3 .. i+2 (slot 5 newest); contravariant v and
+  // covariant u only at rows i-1 .. i+2; blended new height over i-4 .. i+2
+  T wu[6], wv[6], wh[6], wn7[7], wvc[4], wuu[4];
   for (int m = 0; m < 5; ++m) {
-    const size_t o = w.fo + static_cast<size_t>(w.r0 - 3 + m) * NS + jl;
-    CtRaw<T> r;
-    load_ct(a, o, r);
-    wu[m] = ct_u(r);
-    wvc[m] = ct_v(r);
+    BCol<T> r;
+    b_fetch_col<T, GSYM>(a, w, w.r0 - 5 + m, r);
+    wu[m] = (r.g00 * r.u + r.g01 * r.v) * r.sina;
     wv[m] = r.v;
-    wuu[m] = r.u;
-    const T ho = a.h[o];
-    wh[m] = ho;
-    wn7[m + 1] = wn * a.hn[o] + wo * ho;
+    wh[m] = r.h;
+    wn7[m + 1] = wn * r.hn + wo * r.h;
+    if (m >= 2) {
+      wvc[m - 2] = (r.g10 * r.u + r.g11 * r.v) * r.sina;
+      wuu[m - 2] = r.u;
+    }
   }
   wn7[0] = T(0);
-  CtRaw<T> nr;
-  T nh, nhn;
+  BCol<T> pcol;
+  BFace<T> pface;
+  BHalo<T> phalo;
+  BCell<T> pcell;
+  b_fetch_col<T, GSYM>(a, w, w.r0, pcol);
+  b_fetch_face(a, w, w.r0, pface);
+
+  T ucx_prev, vcx_prev, Cx_prev, cdy_prev;
   {
-    const size_t o = w.fo + static_cast<size_t>(w.r0 + 2) * NS + jl;
-    load_ct(a, o, nr);
-    nh = a.h[o];
-    nhn = a.hn[o];
+    const BCol<T> c = pcol;
+    b_fetch_col<T, GSYM>(a, w, w.r0 + 1, pcol);
+    wu[5] = (c.g00 * c.u + c.g01 * c.v) * c.sina;
+    wv[5] = c.v;
+    wh[5] = c.h;
+    wn7[6] = wn * c.hn + wo * c.h;
+    wvc[3] = (c.g10 * c.u + c.g11 * c.v) * c.sina;
+    wuu[3] = c.u;
+    const BFace<T> f = pface;
+    b_fetch_face(a, w, w.r0 + 1, pface);
+    const int i = w.r0;
+    iface_b(wu, wv, wh, (i == HALO) || (i == N - HALO), f.evi, f.dzx, ucx_prev, vcx_prev, Cx_prev);
+    cdy_prev = f.cdy;
+    b_fetch_halo<T, GSYM>(a, w, w.r0, hcol, phalo);
+    b_fetch_cell(a, w, w.r0, pcell);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      wu[m] = wu[m + 1];
+      wv[m] = wv[m + 1];
+      wh[m] = wh[m + 1];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      wvc[m] = wvc[m + 1];
+      wuu[m] = wuu[m + 1];
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) wn7[m] = wn7[m + 1];
   }
-  T n_cdy = P.cdy[w.mo + static_cast<size_t>(w.r0) * NS + jl];
-  T n_dzx = P.dzx_s;
-  if (!P.dz_uniform) n_dzx = P.dzx[w.fo + static_cast<size_t>(w.r0) * NS + jl];
-  T n_rda = T(0), n_ddx = T(0), n_rdx = T(0), n_rdy = T(0), n_af = T(0), n_up = T(0), n_vp = T(0);
-  T n_dzy = P.dzy_s;
-  CtRaw<T> nhr;
-  T nhh = T(0), nhn_h = T(0);
-  T ucx_prev = T(0), vcx_prev = T(0), Cx_prev = T(0), cdy_prev = T(0);
 
-  for (int k = 0; k <= w.r1 - w.r0; ++k) {
-    const int i = w.r0 + k;
-    wu[5] = ct_u(nr);
-    wvc[5] = ct_v(nr);
-    wv[5] = nr.v;
-    wuu[5] = nr.u;
-    wh[5] = nh;
-    wn7[6] = wn * nhn + wo * nh;
-    const T cdy_i = n_cdy, dzx_i = n_dzx;
-    const T rda_c = n_rda, ddx_c = n_ddx, rdx_c = n_rdx, rdy_c = n_rdy, af_c = n_af;
-    const T up_c = n_up, vp_c = n_vp, dzy_c = n_dzy;
-    const CtRaw<T> hr = nhr;
-    const T hh = nhh, hhn = nhn_h;
+  for (int i = w.r0 + 1; i <= w.r1; ++i) {
+    const int ii = i - 1;
     {
-      const int row = min(i + 3, N - 1);
-      const size_t o = w.fo + static_cast<size_t>(row) * NS + jl;
-      load_ct(a, o, nr);
-      nh = a.h[o];
-      nhn = a.hn[o];
-      const int mrow = min(i + 1, N - 1);
-      const size_t om = static_cast<size_t>(mrow) * NS + jl;
-      n_cdy = P.cdy[w.mo + om];
-      if (!P.dz_uniform) n_dzx = P.dzx[w.fo + om];
-      const size_t oc = static_cast<size_t>(i) * NS + jl;
-      n_rda = P.rda[w.mo + oc];
-      n_ddx = P.ddx[w.mo + oc];
-      n_rdx = P.rdx[w.mo + oc];
-      n_rdy = P.rdy[w.mo + oc];
-      n_af = P.af[w.fo + oc];
-      n_up = a.up[w.fo + oc];
-      n_vp = a.vp[w.fo + oc];
-      if (!P.dz_uniform) n_dzy = P.dzy[w.fo + oc];
-      if (halo_lane) {
-        const size_t oh = w.fo + static_cast<size_t>(i) * NS + hcol;
-        load_ct(a, oh, nhr);
-        nhh = a.h[oh];
-        nhn_h = a.hn[oh];
-      }
+      const BCol<T> c = pcol;
+      b_fetch_col<T, GSYM>(a, w, i + 1, pcol);
+      wu[5] = (c.g00 * c.u + c.g01 * c.v) * c.sina;
+      wv[5] = c.v;
+      wh[5] = c.h;
+      wn7[6] = wn * c.hn + wo * c.h;
+      wvc[3] = (c.g10 * c.u + c.g11 * c.v) * c.sina;
+      wuu[3] = c.u;
     }
-
-    // C-grid quantities on the i-face at row i (remap.py:42-95, barotr_pure.py:29-34, 63-75)
-    T ucx = T(0), vcx = T(0), Cx = T(0);
+    T ucx, vcx, Cx, cdy_i;
     {
-      const T ue = e5(wu[0], wu[1], wu[2], wu[3], wu[4]);
-      const T uw = w5(wu[1], wu[2], wu[3], wu[4], wu[5]);
-      const bool edge = ((i == HALO) || (i == N - HALO)) && j <= N - 4;
-      const T upre = edge ? a.evel[(t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO)] : T(0.5) * (ue + uw);
-      vcx = upre > T(0) ? e5(wv[0], wv[1], wv[2], wv[3], wv[4]) : w5(wv[1], wv[2], wv[3], wv[4], wv[5]);
-      const T he = e5(wh[0], wh[1], wh[2], wh[3], wh[4]);
-      const T hw = w5(wh[1], wh[2], wh[3], wh[4], wh[5]);
-      const T cx = celerity(c6(wh[0], wh[1], wh[2], wh[3], wh[4], wh[5]), dzx_i);
-      ucx = upre + T(4.9) / cx * (he - hw);
-      Cx = T(0.5) * cx * (ue - uw);
+      const BFace<T> f = pface;
+      b_fetch_face(a, w, i + 1, pface);
+      iface_b(wu, wv, wh, (i == HALO) || (i == N - HALO), f.evi, f.dzx, ucx, vcx, Cx);
+      cdy_i = f.cdy;
     }
-
-    if (k >= 1) {
-      const int ii = i - 1;
-      bvct[lane + 3] = wvc[2];
-      bu[lane + 3] = wuu[2];
+    // row ii of vct, u, h and the blended new height through LDS
+    {
+      const BHalo<T> hh = phalo;
+      bvct[lane + 3] = wvc[0];
+      bu[lane + 3] = wuu[0];
       bh[lane + 3] = wh[2];
       bn[lane + 3] = wn7[3];
-      if (halo_lane) {
-        bvct[hslot] = ct_v(hr);
-        bu[hslot] = hr.u;
-        bh[hslot] = hh;
-        bn[hslot] = wn * hhn + wo * hh;
-      }
-      wave_lds_sync();
-      T sv[6], su[6], sh[6], sn[7];
+      bvct[hslot] = (hh.g10 * hh.u + hh.g11 * hh.v) * hh.sina;
+      bu[hslot] = hh.u;
+      bh[hslot] = hh.h;
+      bn[hslot] = wn * hh.hn + wo * hh.h;
+      b_fetch_halo<T, GSYM>(a, w, i, hcol, phalo);
+    }
+    wave_lds_sync();
+    T vcy, ucy, Cy, hy0, hy1;
+    const BCell<T> cl = pcell;
+    {
+      T sv[6], su[6], sh[6];
 #pragma unroll
       for (int m = 0; m < 6; ++m) {
         sv[m] = bvct[lane + m];
         su[m] = bu[lane + m];
         sh[m] = bh[lane + m];
-        sn[m] = bn[lane + m];
       }
-      sn[6] = bn[min(lane + 6, kRowBuf - 1)];
-      wave_lds_sync();
       // C-grid quantities on the j-face at column j, row ii
-      T vcy = T(0), ucy = T(0), Cy = T(0);
-      {
-        const T vn = e5(sv[0], sv[1], sv[2], sv[3], sv[4]);
-        const T vs = w5(sv[1], sv[2], sv[3], sv[4], sv[5]);
-        const bool edge = (j == HALO) || (j == N - HALO);
-        const T vpre = edge ? a.evel[(t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO)] : T(0.5) * (vn + vs);
-        ucy = vpre > T(0) ? e5(su[0], su[1], su[2], su[3], su[4]) : w5(su[1], su[2], su[3], su[4], su[5]);
-        const T hn = e5(sh[0], sh[1], sh[2], sh[3], sh[4]);
-        const T hs = w5(sh[1], sh[2], sh[3], sh[4], sh[5]);
-        const T cy = celerity(c6(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]), dzy_c);
-        vcy = vpre + T(4.9) / cy * (hn - hs);
-        Cy = T(0.5) * cy * (vn - vs);
-      }
-      const T ucy_n = __shfl_down(ucy, 1);
-      const T vcy_n = __shfl_down(vcy, 1);
-      const T Cy_n = __shfl_down(Cy, 1);
-      const T ddx_n = __shfl_down(ddx_c, 1);
-      if (lane < kStripOut && j <= N - 4) {
-        // vorticity (agrid.py:32-41) with vv = vb_cx, uu = ub_cy
-        const T zeta = rda_c * (((ucy * ddx_c + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
-        const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
-        const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
-        const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
-        const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
-        const T vct_c = wvc[2];
-        const T uct_c = wu[2];
-        const T advx = zeta * vct_c - (ku1 - ku0) * rdx_c;
-        const T advy = -zeta * uct_c - (kv1 - kv0) * rdy_c;
-        const T visx = -(rdx_c * (Cx - Cx_prev));
-        const T visy = -(rdy_c * (Cy_n - Cy));
-        const T hx0 = c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]);
-        const T hx1 = c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
-        const T hy0 = c6(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]);
-        const T hy1 = c6(sn[1], sn[2], sn[3], sn[4], sn[5], sn[6]);
-        const T pgfu = T(-9.80) * (rdx_c * (hx1 - hx0));
-        const T pgfv = T(-9.80) * (rdy_c * (hy1 - hy0));
-        const size_t oc = w.fo + static_cast<size_t>(ii) * NS + j;
-        a.u_out[oc] = up_c + (((pgfu + advx) + af_c * vct_c) + visx) * a.dt;
-        a.v_out[oc] = vp_c + (((pgfv + advy) - af_c * uct_c) + visy) * a.dt;
-      }
+      iface_b(sv, su, sh, (j == HALO) || (j == N - HALO), cl.evj, cl.dzy, vcy, ucy, Cy);
+      T sn[7];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) sn[m] = bn[min(lane + m, kRowBuf - 1)];
+      hy0 = c6(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]);
+      hy1 = c6(sn[1], sn[2], sn[3], sn[4], sn[5], sn[6]);
     }
+    wave_lds_sync();
+    const T ucy_n = __shfl_down(ucy, 1);
+    const T vcy_n = __shfl_down(vcy, 1);
+    const T Cy_n = __shfl_down(Cy, 1);
+    const T ddx_n = __shfl_down(cl.ddx, 1);
+    // vorticity (agrid.py:32-41) with vv = vb_cx, uu = ub_cy
+    const T zeta = cl.rda * (((ucy * cl.ddx + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
+    const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
+    const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
+    const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
+    const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+    const T vct_c = wvc[0];
+    const T uct_c = wu[2];
+    const T advx = zeta * vct_c - (ku1 - ku0) * cl.rdx;
+    const T advy = -zeta * uct_c - (kv1 - kv0) * cl.rdy;
+    const T visx = -(cl.rdx * (Cx - Cx_prev));
+    const T visy = -(cl.rdy * (Cy_n - Cy));
+    const T hx0 = c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]);
+    const T hx1 = c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
+    const T pgfu = T(-9.80) * (cl.rdx * (hx1 - hx0));
+    const T pgfv = T(-9.80) * (cl.rdy * (hy1 - hy0));
+    const uint32_t oc = w.fo + static_cast<uint32_t>(ii) * NS + j;
+    T* du = out_lane ? at32(a.u_out, oc) : sink;
+    T* dv = out_lane ? at32(a.v_out, oc) : sink;
+    *du = cl.up + (((pgfu + advx) + cl.af * vct_c) + visx) * a.dt;
+    *dv = cl.vp + (((pgfv + advy) - cl.af * uct_c) + visy) * a.dt;
+    b_fetch_cell(a, w, i, pcell);
+
     ucx_prev = ucx;
     vcx_prev = vcx;
     Cx_prev = Cx;
@@ -698,10 +815,13 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
       wu[m] = wu[m + 1];
-      wvc[m] = wvc[m + 1];
       wv[m] = wv[m + 1];
-      wuu[m] = wuu[m + 1];
       wh[m] = wh[m + 1];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      wvc[m] = wvc[m + 1];
+      wuu[m] = wuu[m + 1];
     }
 #pragma unroll
     for (int m = 0; m < 6; ++m) wn7[m] = wn7[m + 1];
@@ -1079,7 +1199,7 @@ struct sw_ctx {
   void* planes = nullptr;  // kNumPlanes T planes
   int* locs = nullptr;     // loc_i, loc_j planes
   void* scratch = nullptr; // 12 T fields: S1, S2 (stage scratch), X, Y (substep ping-pong)
-  void* edges = nullptr;   // evel, eflux, edge-average buffer (24*nx each)
+  void* edges = nullptr;   // evel, eflux, edge-average buffer, store sink
   double dzx_s = 0, dzy_s = 0;
   int dz_uniform = 0;
   int shared = 0;          // rdx, rdy, rda, c_dy, d_dx identical on every face
@@ -1143,9 +1263,16 @@ T* scratch_field(const sw_ctx* c, int k) {  // k in [0, 12)
   return static_cast<T*>(c->scratch) + static_cast<size_t>(k) * c->cells;
 }
 
+// Edge buffers: 0 averaged edge velocity, 1 averaged edge flux, 2 scratch of
+// sw_edge_avg, 3 store sink of the stage kernels (each >= 256 elements).
+size_t edge_stride(const sw_ctx* c) {
+  const size_t n = 24 * static_cast<size_t>(c->nx);
+  return n < 256 ? 256 : n;
+}
+
 template <typename T>
 T* edge_buf(const sw_ctx* c, int k) {
-  return static_cast<T*>(c->edges) + static_cast<size_t>(k) * 24 * c->nx;
+  return static_cast<T*>(c->edges) + static_cast<size_t>(k) * edge_stride(c);
 }
 
 Geo geo_of(const sw_ctx* c) { return Geo{c->nx, c->N}; }
@@ -1202,7 +1329,11 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   ma.nbands = (c->nx + ma.R - 1) / ma.R;
   ma.nwaves = 6 * ma.S * ma.nbands;
   if ((rc = prof_begin(c, K_MARCH_A))) return rc;
-  hipLaunchKernelGGL(k_march_a<T>, dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+  ma.sink = edge_buf<T>(c, 3);
+  if (c->gsym)
+    hipLaunchKernelGGL((k_march_a<T, true>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+  else
+    hipLaunchKernelGGL((k_march_a<T, false>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
   SW_HIP(hipGetLastError());
   if ((rc = prof_end(c))) return rc;
   if ((rc = prof_begin(c, K_HALO_H))) return rc;
@@ -1213,7 +1344,10 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   ma.nbands = (c->nx + ma.R - 1) / ma.R;
   ma.nwaves = 6 * ma.S * ma.nbands;
   if ((rc = prof_begin(c, K_MARCH_B))) return rc;
-  hipLaunchKernelGGL(k_march_b<T>, dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+  if (c->gsym)
+    hipLaunchKernelGGL((k_march_b<T, true>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+  else
+    hipLaunchKernelGGL((k_march_b<T, false>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
   SW_HIP(hipGetLastError());
   if ((rc = prof_end(c))) return rc;
   if ((rc = prof_begin(c, K_HALO_UV))) return rc;
@@ -1390,9 +1524,9 @@ int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
   SW_HIP(hipMalloc(&c->planes, kNumPlanes * n * sizeof(T)));
   SW_HIP(hipMalloc(&c->locs, 2 * n * sizeof(int)));
   SW_HIP(hipMalloc(&c->scratch, 12 * n * sizeof(T)));
-  SW_HIP(hipMalloc(&c->edges, 3 * 24 * static_cast<size_t>(c->nx) * sizeof(T)));
+  SW_HIP(hipMalloc(&c->edges, 4 * edge_stride(c) * sizeof(T)));
   SW_HIP(hipMemsetAsync(c->scratch, 0, 12 * n * sizeof(T), c->stream));
-  SW_HIP(hipMemsetAsync(c->edges, 0, 3 * 24 * static_cast<size_t>(c->nx) * sizeof(T), c->stream));
+  SW_HIP(hipMemsetAsync(c->edges, 0, 4 * edge_stride(c) * sizeof(T), c->stream));
   T* p = static_cast<T*>(c->planes);
   RepackArgs<T> a{};
   a.n = n;
