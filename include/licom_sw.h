@@ -115,6 +115,15 @@ int sw_routing_table(int32_t out[6 * 4 * 3]);
 int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
                double dtb, double ms[6], int counts[6]);
 
+/* Global integrals of a state in binary64 with compensated summation over
+ * the interior cells, whatever the state precision:
+ *   out[0] mass   = sum (dzph + h) * da
+ *   out[1] energy = sum da * (1/2 (dzph + h) u_i G^ij u_j + 1/2 * 9.8 * h^2)
+ * with da = 1 / rda and G^ij = a_gct.  Waits for the result.  Build-defined
+ * diagnostics with no counterpart in the reference (whose diagnostics are
+ * interior max / min / mean, mymodule/diag.py:45-75). */
+int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, double out[2]);
+
 /* Rows per wavefront band of the two stage kernels (performance only; any
  * value in [4, 4096] gives the same results). */
 int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b);

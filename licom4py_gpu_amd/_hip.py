@@ -23,7 +23,7 @@ SW_N_CONSTS = 17
 EXPORTED = (
     "sw_create", "sw_destroy", "sw_set_stream", "sw_rk2", "sw_rk3", "sw_stage",
     "sw_halo_scalar", "sw_halo_vector", "sw_edge_avg", "sw_sync",
-    "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_layout_flags", "sw_profile",
+    "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_layout_flags", "sw_profile", "sw_integrals",
 )
 
 _lib = None
@@ -70,6 +70,8 @@ def lib() -> ctypes.CDLL:
     L.sw_profile.argtypes = [vp, vpp, vpp, ci, ctypes.c_double,
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ci)]
     L.sw_profile.restype = ci
+    L.sw_integrals.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+    L.sw_integrals.restype = ci
     L.sw_set_tuning.argtypes = [vp, ci, ci]
     L.sw_set_tuning.restype = ci
     L.sw_layout_flags.argtypes = [vp, ctypes.POINTER(ci)]

@@ -148,6 +148,16 @@ class SWContext:
         _hip.check(_hip.lib().sw_profile(self._h, pin, pout, int(nbb), float(dtb), ms, cnt), "sw_profile")
         return {name: (ms[k], cnt[k]) for k, name in enumerate(_hip.KERNEL_CLASSES)}
 
+    def integrals(self, h: torch.Tensor, u: torch.Tensor, v: torch.Tensor):
+        """(mass, energy) in binary64 with compensated summation (see
+        include/licom_sw.h); waits for the result."""
+        h, u, v = (self._check_field(x, n) for x, n in ((h, "h"), (u, "u"), (v, "v")))
+        self._bind_stream()
+        out = (ctypes.c_double * 2)()
+        _hip.check(_hip.lib().sw_integrals(self._h, ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(u.data_ptr()),
+                                           ctypes.c_void_p(v.data_ptr()), out), "sw_integrals")
+        return float(out[0]), float(out[1])
+
     def set_tuning(self, rows_a: int, rows_b: int) -> None:
         _hip.check(_hip.lib().sw_set_tuning(self._h, int(rows_a), int(rows_b)), "sw_set_tuning")
 

@@ -1137,6 +1137,101 @@ __global__ __launch_bounds__(256) void k_diag(DiagArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// global integrals in binary64 with compensated (Neumaier) summation
+//   mass   = sum_interior (dzph + h) * da
+//   energy = sum_interior da * (1/2 (dzph + h) |u|^2 + 1/2 g h^2)
+// with da = 1 / rda and |u|^2 = u_i G^{ij} u_j (G^{ij} = a_gct), whatever
+// the precision of the state.
+// ---------------------------------------------------------------------------
+struct Comp {
+  double s, c;
+};
+
+__device__ __forceinline__ void comp_add(Comp& a, double x) {
+  const double t = a.s + x;
+  if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+  else a.c += (x - t) + a.s;
+  a.s = t;
+}
+
+__device__ __forceinline__ Comp comp_merge(Comp a, Comp b) {
+  comp_add(a, b.s);
+  a.c += b.c;
+  return a;
+}
+
+template <typename T>
+struct IntegralArgs {
+  Geo g;
+  Planes<T> P;
+  const T *h, *u, *v;
+  double* partial;  // 4 doubles per block: mass (s, c), energy (s, c)
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_integrals(IntegralArgs<T> a) {
+  const Geo g = a.g;
+  const int nx = g.nx, N = g.N;
+  const size_t cells = 6 * static_cast<size_t>(nx) * nx;
+  Comp m{0.0, 0.0}, e{0.0, 0.0};
+  for (size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < cells;
+       q += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int t = static_cast<int>(q / (static_cast<size_t>(nx) * nx));
+    const int r = static_cast<int>(q % (static_cast<size_t>(nx) * nx));
+    const int i = 3 + r / nx, j = 3 + r % nx;
+    const size_t o = g.at(t, i, j);
+    const double da = 1.0 / static_cast<double>(a.P.rda[o]);
+    const double depth = static_cast<double>(a.P.dz_uniform ? a.P.dzx_s : a.P.dzx[o]) + static_cast<double>(a.h[o]);
+    const double uu = a.u[o], vv = a.v[o];
+    const double q2 = uu * (static_cast<double>(a.P.g00[o]) * uu + static_cast<double>(a.P.g01[o]) * vv) +
+                      vv * (static_cast<double>(a.P.g10[o]) * uu + static_cast<double>(a.P.g11[o]) * vv);
+    const double hh = a.h[o];
+    comp_add(m, depth * da);
+    comp_add(e, da * (0.5 * depth * q2 + 0.5 * 9.8 * hh * hh));
+  }
+  __shared__ Comp sm[256], se[256];
+  sm[threadIdx.x] = m;
+  se[threadIdx.x] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sm[threadIdx.x] = comp_merge(sm[threadIdx.x], sm[threadIdx.x + w]);
+      se[threadIdx.x] = comp_merge(se[threadIdx.x], se[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.partial[4 * blockIdx.x + 0] = sm[0].s;
+    a.partial[4 * blockIdx.x + 1] = sm[0].c;
+    a.partial[4 * blockIdx.x + 2] = se[0].s;
+    a.partial[4 * blockIdx.x + 3] = se[0].c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_integrals_final(const double* partial, int nblocks, double* out) {
+  Comp m{0.0, 0.0}, e{0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    m = comp_merge(m, Comp{partial[4 * b + 0], partial[4 * b + 1]});
+    e = comp_merge(e, Comp{partial[4 * b + 2], partial[4 * b + 3]});
+  }
+  __shared__ Comp sm[256], se[256];
+  sm[threadIdx.x] = m;
+  se[threadIdx.x] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sm[threadIdx.x] = comp_merge(sm[threadIdx.x], sm[threadIdx.x + w]);
+      se[threadIdx.x] = comp_merge(se[threadIdx.x], se[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sm[0].s + sm[0].c;
+    out[1] = se[0].s + se[0].c;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // const repacking (binary64 reference layouts -> context planes of type T)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1200,6 +1295,7 @@ struct sw_ctx {
   int* locs = nullptr;     // loc_i, loc_j planes
   void* scratch = nullptr; // 12 T fields: S1, S2 (stage scratch), X, Y (substep ping-pong)
   void* edges = nullptr;   // evel, eflux, edge-average buffer, store sink
+  double* ibuf = nullptr;  // integral partials (4 per block) and the result
   double dzx_s = 0, dzy_s = 0;
   int dz_uniform = 0;
   int shared = 0;          // rdx, rdy, rda, c_dy, d_dx identical on every face
@@ -1620,7 +1716,26 @@ void free_ctx(sw_ctx* c) {
   if (c->locs) (void)hipFree(c->locs);
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->edges) (void)hipFree(c->edges);
+  if (c->ibuf) (void)hipFree(c->ibuf);
   delete c;
+}
+
+constexpr int kIntegralBlocks = 1024;
+
+template <typename T>
+int run_integrals(sw_ctx* c, const void* h, const void* u, const void* v, double out[2]) {
+  if (!c->ibuf) SW_HIP(hipMalloc(&c->ibuf, (4 * kIntegralBlocks + 2) * sizeof(double)));
+  IntegralArgs<T> a{geo_of(c), planes_of<T>(c), static_cast<const T*>(h), static_cast<const T*>(u),
+                    static_cast<const T*>(v), c->ibuf};
+  hipLaunchKernelGGL(k_integrals<T>, dim3(kIntegralBlocks), dim3(256), 0, c->stream, a);
+  SW_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_integrals_final, dim3(1), dim3(256), 0, c->stream, c->ibuf, kIntegralBlocks,
+                     c->ibuf + 4 * kIntegralBlocks);
+  SW_HIP(hipGetLastError());
+  SW_HIP(hipMemcpyAsync(out, c->ibuf + 4 * kIntegralBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  SW_HIP(hipStreamSynchronize(c->stream));
+  return SW_OK;
 }
 
 }  // namespace
@@ -1773,6 +1888,12 @@ int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nb
   if (rc) return rc;
   if (e != hipSuccess) return fail(SW_E_HIP, std::string("sw_profile: ") + hipGetErrorString(e));
   return SW_OK;
+}
+
+int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, double out[2]) {
+  if (!ctx || !h || !u || !v || !out) return fail(SW_E_ARG, "null argument");
+  return ctx->dtype == SW_F64 ? run_integrals<double>(ctx, h, u, v, out)
+                              : run_integrals<float>(ctx, h, u, v, out);
 }
 
 int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b) {

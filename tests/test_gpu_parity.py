@@ -209,3 +209,85 @@ def test_band_height_does_not_change_results(rows):
     got = ctx.stage(*[_dev(x, torch) for x in core], 20.0, 0.0, True)
     for name, gv, wv in zip(NAMES13, got, want):
         _assert_exact(f"rows={rows} {name}", _host(gv), wv)
+
+
+# ---------------------------------------------------------------------------
+# FP32 form and the binary64 compensated integrals
+# ---------------------------------------------------------------------------
+def _f32_round(arrs):
+    return tuple(np.asarray(a, np.float32).astype(np.float64) for a in arrs)
+
+
+def test_fp32_stage_close_to_oracle():
+    """One FP32 stage vs the FP64 oracle on the same (float32-rounded) inputs:
+    relative L2 <= 1e-5 on every output (FP32 rounding is ~6e-8 per operation)."""
+    torch = _torch()
+    from licom4py_gpu_amd.barotr import SWContext
+    nx = 48
+    _, consts = grid_consts(nx)
+    ctx = SWContext(consts, torch.float32)
+    core = _f32_round(random_core(nx, seed=12))
+    want = B.step_rk_logic(*core, consts, 30.0, 0.0, True)
+    got = ctx.stage(*[_dev(x, torch, torch.float32) for x in core], 30.0, 0.0, True)
+    for name, gv, wv in zip(NAMES13, got, want):
+        err = rel_l2(_host(gv).astype(np.float64), wv)
+        assert err <= 1e-5, f"fp32 stage {name}: rel L2 {err:.3e}"
+
+
+def test_fp32_rk2_c48_vs_fp64_oracle():
+    """FP32 state and stencils, 200 Courant-0.5 substeps at C48: height and
+    velocity relative L2 vs the FP64 oracle within 1e-5."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(48)
+    state = _f32_round(state)
+    want = B.barotr_rk2_core(state, consts, 200, dt)
+    got = barotr.barotr_rk2_core(tuple(_dev(x, torch, torch.float32) for x in state),
+                                 barotr.as_device_consts(consts), 200, dt)
+    for k in (0, 2, 3):
+        err = rel_l2(_host(got[k]).astype(np.float64)[:, 3:-3, 3:-3], want[k][:, 3:-3, 3:-3])
+        assert err <= 1e-5, f"fp32 {B.STATE_KEYS[k]}: rel L2 {err:.3e}"
+
+
+def _integrals_reference(state, consts, dtype):
+    import math
+    h, u, v = (np.asarray(x, dtype).astype(np.float64)[:, 3:-3, 3:-3] for x in (state[0], state[2], state[3]))
+    sl = (slice(None), slice(3, -3), slice(3, -3))
+    rda = np.asarray(consts[14], dtype).astype(np.float64)[sl]
+    dz = np.asarray(consts[0], dtype).astype(np.float64)[sl]
+    G = np.asarray(consts[12], dtype).astype(np.float64)[sl]
+    da = 1.0 / rda
+    depth = dz + h
+    q2 = u * (G[..., 0, 0] * u + G[..., 0, 1] * v) + v * (G[..., 1, 0] * u + G[..., 1, 1] * v)
+    mass = math.fsum((depth * da).ravel())
+    energy = math.fsum((da * (0.5 * depth * q2 + 0.5 * 9.8 * h * h)).ravel())
+    return mass, energy
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_integrals_match_exact_sum(dtype):
+    torch = _torch()
+    from licom4py_gpu_amd.barotr import SWContext
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    ndt = np.float64 if dtype == "f64" else np.float32
+    state, consts, dt = seeded_state(96)
+    ctx = SWContext(consts, tdt)
+    m, e = ctx.integrals(*(_dev(np.asarray(state[k], ndt), torch) for k in (0, 2, 3)))
+    rm, re_ = _integrals_reference(state, consts, ndt)
+    assert abs(m - rm) <= 1e-14 * abs(rm)
+    assert abs(e - re_) <= 1e-13 * abs(re_)
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 1e-6)])
+def test_mass_drift_small(dtype, tol):
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    state, consts, dt = seeded_state(48)
+    dconsts = barotr.as_device_consts(consts)
+    ctx = barotr.context_for(dconsts, tdt)
+    s = tuple(_dev(x, torch, tdt) for x in state)
+    m0, _ = ctx.integrals(s[0], s[2], s[3])
+    s = barotr.barotr_rk2_core(s, dconsts, 100, dt)
+    m1, _ = ctx.integrals(s[0], s[2], s[3])
+    assert abs(m1 - m0) / m0 <= tol
