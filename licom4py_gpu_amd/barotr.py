@@ -99,8 +99,9 @@ class SWContext:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value and _hip._lib is not None:
-            _hip.lib().sw_destroy(h)
+        lib = getattr(_hip, "_lib", None) if _hip is not None else None  # None at interpreter exit
+        if h is not None and h.value and lib is not None:
+            lib.sw_destroy(h)
             self._h = None
 
     # -- helpers -----------------------------------------------------------

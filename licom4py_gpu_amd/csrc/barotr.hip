@@ -286,10 +286,18 @@ __global__ __launch_bounds__(256) void k_edge(EdgeArgs<T> a) {
 // the right, loaded by lanes 0..4).  A wave computes j-faces at its 64
 // columns and writes cells c0 .. c0+62, so strips start every 63 columns.
 // The next row's loads are issued one step ahead of their use.  Waves are
-// independent (no workgroup barrier); the six faces of one (strip, band) are
-// adjacent wave ids, so the face-independent metric planes they share are
-// served from cache.
+// independent (no workgroup barrier).
+//
+// Tile order.  The dispatcher hands workgroup b to XCD b % 8, and each XCD has
+// its own L2.  decode_wave turns the hardware index into a logical one that is
+// contiguous per XCD, and logical wave ids run face fastest, then strip, then
+// band: the six faces of one (strip, band) - which read the same
+// face-independent metric planes - and the neighbouring strips of a band -
+// which share the 6 overlap columns and the 128-byte lines that straddle strip
+// boundaries - are resident on one XCD at the same time and fetch those lines
+// from HBM once.
 // ---------------------------------------------------------------------------
+constexpr int kXcds = 8;
 constexpr int kStripOut = 63;  // cells written per wave and row
 constexpr int kRowBuf = 72;    // LDS row buffer length (69 used)
 
@@ -336,12 +344,16 @@ __device__ __forceinline__ bool decode_wave(const MarchArgs<T>& a, WaveTile& w) 
   const int N = a.g.N;
   w.lane = threadIdx.x & 63;
   w.wslot = threadIdx.x >> 6;
-  const int wid = blockIdx.x * (blockDim.x >> 6) + w.wslot;
+  // bijection blockIdx -> logical workgroup, contiguous per XCD
+  const int G = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+  const int x = b % kXcds, q = G / kXcds, r = G % kXcds;
+  const int L = x * q + min(x, r) + b / kXcds;
+  const int wid = L * (blockDim.x >> 6) + w.wslot;
   if (wid >= a.nwaves) return false;
   w.t = wid % 6;
   const int rest = wid / 6;
-  const int band = rest % a.nbands;
-  const int s = rest / a.nbands;
+  const int s = rest % a.S;
+  const int band = rest / a.S;
   w.c0 = HALO + kStripOut * s;
   w.j = w.c0 + w.lane;
   w.jl = min(w.j, N - 1);

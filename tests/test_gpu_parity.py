@@ -211,6 +211,43 @@ def test_band_height_does_not_change_results(rows):
         _assert_exact(f"rows={rows} {name}", _host(gv), wv)
 
 
+def _generic_consts(consts, which):
+    """Copies of the consts with one layout property the kernels exploit broken."""
+    c = [np.array(x, copy=True) for x in consts]
+    nx = c[0].shape[-1] - 6
+    if "dz" in which:      # dzph not uniform -> per-cell planes
+        r = np.random.default_rng(5)
+        c[0] = c[0] + r.uniform(-20, 20, size=c[0].shape)
+        c[1] = c[1] + r.uniform(-20, 20, size=c[1].shape)
+    if "shared" in which:  # rdx differs on face 2 -> per-face metric planes
+        c[2][2] *= 1.0 + 1e-3
+    if "gsym" in which:    # a_gct not symmetric on face 3
+        c[12][3, ..., 0, 1] *= 1.0 + 1e-3
+    return tuple(c)
+
+
+@pytest.mark.parametrize("which,flags", [
+    (("dz", "shared", "gsym"), (False, False, False)),
+    (("shared",), (True, False, True)),
+])
+@pytest.mark.parametrize("last", [False, True])
+def test_generic_layout_matches_oracle(which, flags, last):
+    """Kernel paths for consts without the grid's uniform / shared structure."""
+    torch = _torch()
+    from licom4py_gpu_amd.barotr import SWContext
+    nx = 13
+    _, base = grid_consts(nx)
+    consts = _generic_consts(base, which)
+    ctx = SWContext(consts, torch.float64)
+    f = ctx.layout_flags()
+    assert (f["dz_uniform"], f["shared_metrics"], f["gct_symmetric"]) == flags
+    core = random_core(nx, seed=11 + int(last))
+    want = B.step_rk_logic(*core, consts, 30.0, 0.5, last)
+    got = ctx.stage(*[_dev(x, torch) for x in core], 30.0, 0.5, last)
+    for name, gv, wv in zip(NAMES13, got, want):
+        _assert_exact(f"{which} {name}", _host(gv), wv)
+
+
 # ---------------------------------------------------------------------------
 # FP32 form and the binary64 compensated integrals
 # ---------------------------------------------------------------------------
