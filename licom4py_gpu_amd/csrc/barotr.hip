@@ -249,26 +249,34 @@ struct EdgeArgs {
   const T *h, *u, *v;
   T *evel, *eflux;
   int last;
+  int ue[12];  // the 12 cube edges, each once: t * 4 + d with (t, d) < its neighbour
 };
 
+// Edge values of both faces meeting at a cube edge (communication.py:485-564
+// averaging of the C-grid normal velocity and, on the last stage, of the
+// upwind flux).  Lanes 2k and 2k+1 take the two sides of one edge point and
+// swap their one-sided values, so every cube edge is evaluated once and both
+// faces' entries of evel / eflux (index (t * 4 + d) * nx + r) are written.
+// IEEE addition is commutative, so (own + other) / 2 equals the reference's
+// (mine + nb) / 2 seen from either face.
 template <typename T>
 __global__ __launch_bounds__(256) void k_edge(EdgeArgs<T> a) {
   const int nx = a.g.nx;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 24 * nx) return;
-  const int t = q / (4 * nx);
-  const int d = (q / nx) % 4;
-  const int r = q % nx;
+  if (q >= 24 * nx) return;  // 24 nx is even: lane pairs stay whole
+  const int p = q >> 1, side = q & 1;
+  const int e = p / nx, r = p - e * nx;
+  const int t = a.ue[e] >> 2, d = a.ue[e] & 3;
   const int tb = a.R.r[t][d][0], db = a.R.r[t][d][1];
   const int rr = a.R.r[t][d][2] ? nx - 1 - r : r;
-  const T mine = edge_normal_velocity(a.g, a.P, a.u, a.v, t, d, r);
-  const T nb = edge_normal_velocity(a.g, a.P, a.u, a.v, tb, db, rr);
-  const T avg = (mine + nb) / T(2.0);
-  a.evel[q] = avg;
+  const int ts = side ? tb : t, ds = side ? db : d, rs = side ? rr : r;
+  const T vel = edge_normal_velocity(a.g, a.P, a.u, a.v, ts, ds, rs);
+  const T avg = (vel + __shfl_xor(vel, 1)) / T(2.0);
+  const int o = (ts * 4 + ds) * nx + rs;
+  a.evel[o] = avg;
   if (a.last) {
-    const T fm = edge_flux(a.g, a.P, a.h, t, d, r, avg);
-    const T fn = edge_flux(a.g, a.P, a.h, tb, db, rr, avg);
-    a.eflux[q] = (fm + fn) / T(2.0);
+    const T f = edge_flux(a.g, a.P, a.h, ts, ds, rs, avg);
+    a.eflux[o] = (f + __shfl_xor(f, 1)) / T(2.0);
   }
 }
 
@@ -913,64 +921,120 @@ __device__ __forceinline__ void halo_cell(int N, int nx, int q, int& i, int& j, 
   }
 }
 
+// Halo plan.  Every halo cell's gathers are resolved once, at context
+// creation: entry q holds the cell's offset, its two interpolation sources and
+// weights, and - for vector fields - the rotation coefficients the remap
+// reads at the two sources (cube to lat-lon) and at the cell itself (lat-lon
+// to cube).  They are copies of const entries, so the planned fill evaluates
+// exactly the expressions of the direct one while reading a few contiguous
+// arrays instead of scattered lines of ten (6, N, N) planes.  Entries of the
+// W/E strips run layer fastest: three neighbouring lanes write three adjacent
+// cells of one row.
+constexpr int kPlanRot = 12;
+
 template <typename T>
-struct HaloArgs {
-  Geo g;
-  Route R;
-  Planes<T> P;
-  T *f;        // scalar field base (6, ncomp, N, N)
-  int ncomp;
-  T *u, *v;    // vector pair
+struct HaloPlan {
+  const uint32_t *dst, *s0, *s1;
+  const T *c0, *c1;
+  const T* rot;  // kPlanRot x n: c2l(s0) 00 01 10 11, c2l(s1) 00 01 10 11, l2c(dst) 00 01 10 11
+  int n;
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_halo_scalar(HaloArgs<T> a) {
+struct PlanBuildArgs {
+  Geo g;
+  Route R;
+  Planes<T> P;
+  uint32_t *dst, *s0, *s1;
+  T *c0, *c1, *rot;
+  int n;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_plan_build(PlanBuildArgs<T> a) {
   const Geo g = a.g;
-  const int per_face = 12 * g.nx + 36;
+  const int nx = g.nx, per_face = 12 * nx + 36;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 6 * per_face) return;
+  if (q >= a.n) return;
   const int t = q / per_face;
+  int qf = q - t * per_face;
+  if (qf < 6 * nx) {  // W/E strips: entry order (d, r, layer) -> halo_cell order (d, layer, r)
+    const int d = qf / (3 * nx), rem = qf - d * 3 * nx;
+    qf = d * 3 * nx + (rem % 3) * nx + rem / 3;
+  }
   int i, j, si, sj;
-  halo_cell(g.N, g.nx, q - t * per_face, i, j, si, sj);
+  halo_cell(g.N, nx, qf, i, j, si, sj);
   size_t s0, s1;
   T c0, c1;
   strip_sources(g, a.R, a.P, t, si, sj, s0, s1, c0, c1);
-  const size_t NN = static_cast<size_t>(g.N) * g.N;
-  for (int comp = 0; comp < a.ncomp; ++comp) {
-    // face-major stacking: face f, component comp at (f * ncomp + comp) * NN
-    auto off = [&](size_t o) {
-      const size_t f = o / NN;
-      return (f * a.ncomp + comp) * NN + (o - f * NN);
-    };
-    const T val = c0 * a.f[off(s0)] + c1 * a.f[off(s1)];
-    a.f[off(g.at(t, i, j))] = val;
-  }
+  const size_t o = g.at(t, i, j);
+  a.dst[q] = static_cast<uint32_t>(o);
+  a.s0[q] = static_cast<uint32_t>(s0);
+  a.s1[q] = static_cast<uint32_t>(s1);
+  a.c0[q] = c0;
+  a.c1[q] = c1;
+  const Planes<T>& P = a.P;
+  const T r[kPlanRot] = {P.c00[s0], P.c01[s0], P.c10[s0], P.c11[s0], P.c00[s1], P.c01[s1],
+                         P.c10[s1], P.c11[s1], P.l00[o], P.l01[o], P.l10[o], P.l11[o]};
+#pragma unroll
+  for (int k = 0; k < kPlanRot; ++k) a.rot[static_cast<size_t>(k) * a.n + q] = r[k];
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
-  const Geo g = a.g;
-  const int per_face = 12 * g.nx + 36;
+struct HaloArgs {
+  HaloPlan<T> plan;
+  uint32_t NN;  // N * N
+  T *f;         // scalar field base (6, ncomp, N, N)
+  int ncomp;
+  T *u, *v;     // vector pair
+};
+
+// Scalar halo fill (update_domain pass 1 + cube_rmp, communication.py:404-430,
+// cube.py:64-125) from the plan.
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_scalar(HaloArgs<T> a) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 6 * per_face) return;
-  const int t = q / per_face;
-  int i, j, si, sj;
-  halo_cell(g.N, g.nx, q - t * per_face, i, j, si, sj);
-  size_t s0, s1;
-  T c0, c1;
-  strip_sources(g, a.R, a.P, t, si, sj, s0, s1, c0, c1);
-  const Planes<T>& P = a.P;
-  // rotate the copied interior values to lat-lon (cube.py:139-140; inner = 1)
-  const T ul0 = P.c00[s0] * a.u[s0] + P.c01[s0] * a.v[s0];
-  const T vl0 = P.c10[s0] * a.u[s0] + P.c11[s0] * a.v[s0];
-  const T ul1 = P.c00[s1] * a.u[s1] + P.c01[s1] * a.v[s1];
-  const T vl1 = P.c10[s1] * a.u[s1] + P.c11[s1] * a.v[s1];
+  if (q >= a.plan.n) return;
+  const uint32_t d = a.plan.dst[q], s0 = a.plan.s0[q], s1 = a.plan.s1[q];
+  const T c0 = a.plan.c0[q], c1 = a.plan.c1[q];
+  if (a.ncomp == 1) {
+    a.f[d] = c0 * a.f[s0] + c1 * a.f[s1];
+    return;
+  }
+  // face-major stacking: face f, component comp at (f * ncomp + comp) * NN
+  const uint32_t NN = a.NN;
+  auto off = [&](uint32_t o, int comp) {
+    const size_t f = o / NN;
+    return (f * a.ncomp + comp) * NN + (o - f * NN);
+  };
+  for (int comp = 0; comp < a.ncomp; ++comp) {
+    const T val = c0 * a.f[off(s0, comp)] + c1 * a.f[off(s1, comp)];
+    a.f[off(d, comp)] = val;
+  }
+}
+
+// Vector halo fill from the plan: the copied interior values rotated to
+// lat-lon (cube.py:139-140; inner = 1), interpolated, and rotated back with
+// the cell's own rotation (cube.py:147-151; outer = 1).
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.plan.n;
+  if (q >= n) return;
+  const uint32_t d = a.plan.dst[q], s0 = a.plan.s0[q], s1 = a.plan.s1[q];
+  const T c0 = a.plan.c0[q], c1 = a.plan.c1[q];
+  T R[kPlanRot];
+#pragma unroll
+  for (int k = 0; k < kPlanRot; ++k) R[k] = a.plan.rot[static_cast<size_t>(k) * n + q];
+  const T u0 = a.u[s0], v0 = a.v[s0], u1 = a.u[s1], v1 = a.v[s1];
+  const T ul0 = R[0] * u0 + R[1] * v0;
+  const T vl0 = R[2] * u0 + R[3] * v0;
+  const T ul1 = R[4] * u1 + R[5] * v1;
+  const T vl1 = R[6] * u1 + R[7] * v1;
   const T p0 = c0 * ul0 + c1 * ul1;
   const T p1 = c0 * vl0 + c1 * vl1;
-  // back to covariant with this cell's own rotation (cube.py:147-151; outer = 1)
-  const size_t o = g.at(t, i, j);
-  a.u[o] = P.l00[o] * p0 + P.l01[o] * p1;
-  a.v[o] = P.l10[o] * p0 + P.l11[o] * p1;
+  a.u[d] = R[8] * p0 + R[9] * p1;
+  a.v[d] = R[10] * p0 + R[11] * p1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1312,6 +1376,8 @@ struct sw_ctx {
   int dz_uniform = 0;
   int shared = 0;          // rdx, rdy, rda, c_dy, d_dx identical on every face
   int gsym = 0;            // a_gct symmetric with equal diagonal everywhere
+  void* plan = nullptr;    // halo plan (uint32 dst, s0, s1; T c0, c1, rot[12]), plan_n entries
+  int plan_n = 0;
   int rows_a = 32;         // rows per wave of k_march_a
   int rows_b = 32;         // rows per wave of k_march_b
   hipStream_t capture = nullptr;  // private stream used only to record graphs
@@ -1385,20 +1451,57 @@ T* edge_buf(const sw_ctx* c, int k) {
 
 Geo geo_of(const sw_ctx* c) { return Geo{c->nx, c->N}; }
 
+// Plan arrays inside the one allocation c->plan: three uint32 arrays, then
+// (2 + kPlanRot) arrays of T, each starting on a 256-byte boundary.
+size_t plan_stride(size_t n, size_t esize) { return ((n * esize + 255) / 256) * 256; }
+
+template <typename T>
+HaloPlan<T> plan_of(const sw_ctx* c) {
+  const size_t n = static_cast<size_t>(c->plan_n);
+  char* b = static_cast<char*>(c->plan);
+  const size_t su = plan_stride(n, 4), st = plan_stride(n, sizeof(T));
+  HaloPlan<T> p;
+  p.dst = reinterpret_cast<const uint32_t*>(b);
+  p.s0 = reinterpret_cast<const uint32_t*>(b + su);
+  p.s1 = reinterpret_cast<const uint32_t*>(b + 2 * su);
+  p.c0 = reinterpret_cast<const T*>(b + 3 * su);
+  p.c1 = reinterpret_cast<const T*>(b + 3 * su + st);
+  p.rot = reinterpret_cast<const T*>(b + 3 * su + 2 * st);
+  p.n = c->plan_n;
+  return p;
+}
+
+template <typename T>
+int build_plan(sw_ctx* c) {
+  const size_t n = 6 * (12 * static_cast<size_t>(c->nx) + 36);
+  const size_t su = plan_stride(n, 4), st = plan_stride(n, sizeof(T));
+  // the rot arrays are addressed k * n + q from p.rot: keep them n apart
+  const size_t bytes = 3 * su + 2 * st + kPlanRot * n * sizeof(T) + 256;
+  SW_HIP(hipMalloc(&c->plan, bytes));
+  c->plan_n = static_cast<int>(n);
+  const HaloPlan<T> p = plan_of<T>(c);
+  PlanBuildArgs<T> a{geo_of(c), route(), planes_of<T>(c), const_cast<uint32_t*>(p.dst),
+                     const_cast<uint32_t*>(p.s0), const_cast<uint32_t*>(p.s1), const_cast<T*>(p.c0),
+                     const_cast<T*>(p.c1), const_cast<T*>(p.rot), c->plan_n};
+  hipLaunchKernelGGL(k_plan_build<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+  SW_HIP(hipGetLastError());
+  return SW_OK;
+}
+
 template <typename T>
 int launch_halo_scalar(const sw_ctx* c, T* f, int ncomp) {
-  HaloArgs<T> a{geo_of(c), route(), planes_of<T>(c), f, ncomp, nullptr, nullptr};
-  const size_t n = 6 * (12 * static_cast<size_t>(c->nx) + 36);
-  hipLaunchKernelGGL(k_halo_scalar<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+  const uint32_t NN = static_cast<uint32_t>(c->N) * static_cast<uint32_t>(c->N);
+  HaloArgs<T> a{plan_of<T>(c), NN, f, ncomp, nullptr, nullptr};
+  hipLaunchKernelGGL(k_halo_scalar<T>, dim3(blocks_for(c->plan_n)), dim3(256), 0, c->stream, a);
   SW_HIP(hipGetLastError());
   return SW_OK;
 }
 
 template <typename T>
 int launch_halo_vector(const sw_ctx* c, T* u, T* v) {
-  HaloArgs<T> a{geo_of(c), route(), planes_of<T>(c), nullptr, 1, u, v};
-  const size_t n = 6 * (12 * static_cast<size_t>(c->nx) + 36);
-  hipLaunchKernelGGL(k_halo_vector<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+  const uint32_t NN = static_cast<uint32_t>(c->N) * static_cast<uint32_t>(c->N);
+  HaloArgs<T> a{plan_of<T>(c), NN, nullptr, 1, u, v};
+  hipLaunchKernelGGL(k_halo_vector<T>, dim3(blocks_for(c->plan_n)), dim3(256), 0, c->stream, a);
   SW_HIP(hipGetLastError());
   return SW_OK;
 }
@@ -1412,7 +1515,15 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   const Planes<T> P = planes_of<T>(c);
   T* evel = edge_buf<T>(c, 0);
   T* eflux = edge_buf<T>(c, 1);
-  EdgeArgs<T> ea{g, route(), P, h, u, v, evel, eflux, last};
+  EdgeArgs<T> ea{g, route(), P, h, u, v, evel, eflux, last, {}};
+  {
+    const Route R = route();
+    int k = 0;
+    for (int t = 0; t < 6; ++t)
+      for (int d = 0; d < 4; ++d)
+        if (t * 4 + d < R.r[t][d][0] * 4 + R.r[t][d][1] && k < 12) ea.ue[k++] = t * 4 + d;
+    if (k != 12) return fail(SW_E_STATE, "routing table does not pair the 24 face edges");
+  }
   int rc = prof_begin(c, K_EDGE);
   if (rc) return rc;
   hipLaunchKernelGGL(k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), dim3(256), 0,
@@ -1661,7 +1772,7 @@ int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
   a.o_lj = c->locs + n;
   hipLaunchKernelGGL(k_repack<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
   SW_HIP(hipGetLastError());
-  return SW_OK;
+  return build_plan<T>(c);
 }
 
 // Host-side validation of the tables the kernels index with.
@@ -1729,6 +1840,7 @@ void free_ctx(sw_ctx* c) {
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->edges) (void)hipFree(c->edges);
   if (c->ibuf) (void)hipFree(c->ibuf);
+  if (c->plan) (void)hipFree(c->plan);
   delete c;
 }
 
