@@ -12,7 +12,25 @@ Arrays carry the face axis first: (6, N, N) with N = nx + 6, or (6, 2, N, N)
 for the stacked vector halo exchange.  NumPy never fuses multiply-add, so every
 expression below is evaluated with the reference's association order.
 
-Parity status of each part (how it is pinned) is listed in tests/README.md.
+How each part is pinned (tests/test_oracle_pins.py unless noted).  The
+reference ships no golden vectors and its JAX path cannot run here, so the
+pins are known answers and what the mathematics fixes:
+  routing_table / cube_edges   known-answer table, cross-checked against the
+                               Fortran neighbour rule; involution
+  update_domain (halo copy)    unique-id fill checked against the closed form
+  cube_rmp / ext_scalar        second-order reproduction of a smooth field;
+                               per-point loop restatement (loop_ref.py)
+  ext_vector                   loop restatement; rotation pairs inverse
+                               (tests/test_grid.py)
+  Poly (vector_*, scalar_*)    exact on polynomial cell averages (degree 4 / 5)
+  boundary_communication       both sides of every cube edge equal afterwards
+  step_rk_logic (all operators)  per-point loop restatement bit-identical at
+                               C8 / C10; resting state stays at rest; mass
+                               conserved to round-off with flux averaging
+  barotr_rk2_core / rk3_core   two (three) explicit stages
+Parity unpinned (no import or fixture can fix them here): the rounding
+sequence of the reference's XLA CPU trajectory (FMA contraction or reordered
+sums), for which the 1e-12 relative-L2 bar is the tolerance.
 """
 
 from __future__ import annotations

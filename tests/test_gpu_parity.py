@@ -91,6 +91,22 @@ def test_edge_avg_matches_oracle(nx):
     _assert_exact("edge avg v", _host(gv), wv)
 
 
+def test_stage_matches_golden_fixture():
+    """One C8 stage against the frozen oracle output in tests/golden/stage_c8.npz."""
+    import os
+    torch = _torch()
+    from licom4py_gpu_amd.barotr import SWContext
+    path = os.path.join(os.path.dirname(__file__), "golden", "stage_c8.npz")
+    with np.load(path, allow_pickle=False) as z:
+        core = [z[f"in_{k}"] for k in ("h", "hp", "u", "v", "up", "vp")]
+        want = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+        dt, beta, last = float(z["dt"]), float(z["beta"]), bool(z["last"])
+    _, consts = grid_consts(8)
+    got = SWContext(consts, torch.float64).stage(*[_dev(x, torch) for x in core], dt, beta, last)
+    for name, gv in zip(NAMES13, got):
+        _assert_exact(f"golden {name}", _host(gv), want[name])
+
+
 @pytest.mark.parametrize("nx", [8, 13, 48, 70])
 @pytest.mark.parametrize("last", [False, True])
 @pytest.mark.parametrize("beta", [0.0, 1.0])

@@ -247,3 +247,20 @@ def test_rk2_is_two_stages():
     assert np.array_equal(out[2], r2[1]) and np.array_equal(out[5], r2[2])
     for k in range(3, 13):
         assert np.array_equal(out[k + 3], r2[k])
+
+
+def test_oracle_reproduces_golden_stage():
+    """The oracle still produces the committed C8 stage (tests/golden/stage_c8.npz)."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "stage_c8.npz")
+    with np.load(path, allow_pickle=False) as z:
+        core = [z[f"in_{k}"] for k in ("h", "hp", "u", "v", "up", "vp")]
+        want = [z[k] for k in z.files if k.startswith("out_")]
+        names = [k for k in z.files if k.startswith("out_")]
+        dt, beta, last = float(z["dt"]), float(z["beta"]), bool(z["last"])
+    _, consts = grid_consts(8)
+    got = B.step_rk_logic(*core, consts, dt, beta, last)
+    order = ("h0", "ub", "vb", "celerity_x", "celerity_y", "ub_ct", "vb_ct",
+             "ub_cx", "ub_cy", "vb_cx", "vb_cy", "advx", "advy")
+    for k, g in zip(order, got):
+        assert np.array_equal(g, want[names.index(f"out_{k}")]), k
