@@ -133,6 +133,37 @@ int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b);
  * faces (read from face 0); bit 2: a_gct symmetric with equal diagonal. */
 int sw_layout_flags(const sw_ctx* ctx, int* flags);
 
+/* A context for another member of the same resolution: it shares base's
+ * device copy of the consts (kept alive until the last context using it is
+ * destroyed) and has its own workspace.  For ensembles and served solves; no
+ * counterpart in the reference, which runs one field per process. */
+int sw_create_like(sw_ctx** out, const sw_ctx* base);
+
+/* One member of a batched run: its own context, the six input fields and the
+ * 13 outputs (as for sw_rk2), its substep count and substep length. */
+typedef struct sw_member {
+  sw_ctx* ctx;
+  const void* in[6];
+  void* out[13];
+  int nbb;
+  double dtb;
+} sw_member;
+
+typedef struct sw_batch sw_batch;
+
+/* Reusable device tables of batched runs. */
+int sw_batch_create(sw_batch** out);
+void sw_batch_destroy(sw_batch* b);
+
+/* Advance `count` members (any resolutions, one dtype, distinct contexts) by
+ * their own nbb RK substeps (rk = 2 or 3), every kernel class launched once
+ * per stage for all members still stepping, on `stream`.  Each member's
+ * outputs equal those of sw_rk2 / sw_rk3 on its context alone (flags:
+ * SW_SKIP_DIAG).  Generalises the reference's one-field loop
+ * (momentum/barotr.py:41-87) to many fields per launch.  A batch object may be
+ * reused; its tables are replaced at every run. */
+int sw_batch_run(sw_batch* b, const sw_member* members, int count, int rk, int flags, void* stream);
+
 /* Host-only: resolution and dtype of a context. */
 int sw_info(const sw_ctx* ctx, int* nx, int* dtype);
 

@@ -24,7 +24,14 @@ EXPORTED = (
     "sw_create", "sw_destroy", "sw_set_stream", "sw_rk2", "sw_rk3", "sw_stage",
     "sw_halo_scalar", "sw_halo_vector", "sw_edge_avg", "sw_sync",
     "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_layout_flags", "sw_profile", "sw_integrals",
+    "sw_create_like", "sw_batch_create", "sw_batch_destroy", "sw_batch_run",
 )
+
+
+class SWMember(ctypes.Structure):
+    """``sw_member`` of include/licom_sw.h."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("inp", ctypes.c_void_p * 6), ("out", ctypes.c_void_p * 13),
+                ("nbb", ctypes.c_int), ("dtb", ctypes.c_double)]
 
 _lib = None
 
@@ -76,6 +83,14 @@ def lib() -> ctypes.CDLL:
     L.sw_set_tuning.restype = ci
     L.sw_layout_flags.argtypes = [vp, ctypes.POINTER(ci)]
     L.sw_layout_flags.restype = ci
+    L.sw_create_like.argtypes = [ctypes.POINTER(vp), vp]
+    L.sw_create_like.restype = ci
+    L.sw_batch_create.argtypes = [ctypes.POINTER(vp)]
+    L.sw_batch_create.restype = ci
+    L.sw_batch_destroy.argtypes = [vp]
+    L.sw_batch_destroy.restype = None
+    L.sw_batch_run.argtypes = [vp, ctypes.POINTER(SWMember), ci, ci, ci, vp]
+    L.sw_batch_run.restype = ci
     L.sw_last_error.argtypes = []
     L.sw_last_error.restype = ctypes.c_char_p
     _lib = L

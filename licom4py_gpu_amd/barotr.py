@@ -79,6 +79,8 @@ class SWContext:
     precision.  Launches go to the current torch stream of the call."""
 
     def __init__(self, consts: Sequence, dtype: torch.dtype = torch.float64):
+        if consts is None:  # filled in by like()
+            return
         if dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be torch.float64 or torch.float32")
         dc = as_device_consts(consts)
@@ -96,6 +98,16 @@ class SWContext:
                                          _hip.SW_F64 if dtype == torch.float64 else _hip.SW_F32,
                                          arr, ctypes.c_void_p(stream)), "sw_create")
         self._h = handle
+
+    def like(self) -> "SWContext":
+        """Another context of this resolution and precision with its own
+        workspace; the device copy of the consts is shared."""
+        other = SWContext(None)
+        other.nx, other.N, other.dtype, other.device = self.nx, self.N, self.dtype, self.device
+        handle = ctypes.c_void_p()
+        _hip.check(_hip.lib().sw_create_like(ctypes.byref(handle), self._h), "sw_create_like")
+        other._h = handle
+        return other
 
     def __del__(self):
         h = getattr(self, "_h", None)

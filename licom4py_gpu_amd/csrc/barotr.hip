@@ -33,7 +33,11 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
+#include <type_traits>
 #include <cstring>
+#include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -260,9 +264,8 @@ struct EdgeArgs {
 // IEEE addition is commutative, so (own + other) / 2 equals the reference's
 // (mine + nb) / 2 seen from either face.
 template <typename T>
-__global__ __launch_bounds__(256) void k_edge(EdgeArgs<T> a) {
+__device__ __forceinline__ void edge_body(const EdgeArgs<T>& a, int q) {
   const int nx = a.g.nx;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= 24 * nx) return;  // 24 nx is even: lane pairs stay whole
   const int p = q >> 1, side = q & 1;
   const int e = p / nx, r = p - e * nx;
@@ -278,6 +281,11 @@ __global__ __launch_bounds__(256) void k_edge(EdgeArgs<T> a) {
     const T f = edge_flux(a.g, a.P, a.h, ts, ds, rs, avg);
     a.eflux[o] = (f + __shfl_xor(f, 1)) / T(2.0);
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge(EdgeArgs<T> a) {
+  edge_body(a, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -347,16 +355,20 @@ struct WaveTile {
   uint32_t fo, mo;  // element offsets (fields stay below 2^29 elements)
 };
 
+// Bijection blockIdx -> logical workgroup index, contiguous per XCD.
+__device__ __forceinline__ int xcd_block() {
+  const int G = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+  const int x = b % kXcds, q = G / kXcds, r = G % kXcds;
+  return x * q + min(x, r) + b / kXcds;
+}
+
+// Wave of logical block L (four waves per block) -> its (face, strip, band).
 template <typename T>
-__device__ __forceinline__ bool decode_wave(const MarchArgs<T>& a, WaveTile& w) {
+__device__ __forceinline__ bool decode_wave(const MarchArgs<T>& a, int L, WaveTile& w) {
   const int N = a.g.N;
   w.lane = threadIdx.x & 63;
   w.wslot = threadIdx.x >> 6;
-  // bijection blockIdx -> logical workgroup, contiguous per XCD
-  const int G = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
-  const int x = b % kXcds, q = G / kXcds, r = G % kXcds;
-  const int L = x * q + min(x, r) + b / kXcds;
-  const int wid = L * (blockDim.x >> 6) + w.wslot;
+  const int wid = L * 4 + w.wslot;
   if (wid >= a.nwaves) return false;
   w.t = wid % 6;
   const int rest = wid / 6;
@@ -466,11 +478,10 @@ __device__ __forceinline__ T face_flux2(const T* s, const T* w, bool edge, T upr
 }
 
 template <typename T, bool GSYM>
-__global__ __launch_bounds__(256) void k_march_a(MarchArgs<T> a) {
-  __shared__ T s_v[4][kRowBuf];
-  __shared__ T s_h[4][kRowBuf];
+__device__ __forceinline__ void march_a_body(const MarchArgs<T>& a, int L, T (*s_v)[kRowBuf],
+                                             T (*s_h)[kRowBuf]) {
   WaveTile w;
-  if (!decode_wave(a, w)) return;
+  if (!decode_wave(a, L, w)) return;
   const int N = a.g.N;
   T* bv = s_v[w.wslot];
   T* bh = s_h[w.wslot];
@@ -482,7 +493,7 @@ __global__ __launch_bounds__(256) void k_march_a(MarchArgs<T> a) {
   hcol = halo_lane ? min(hcol, N - 1) : w.jl;
   const int hslot = halo_lane ? hcol - c0 + 3 : lane + 3;  // others rewrite their own slot
   const bool out_lane = lane < kStripOut && j <= N - 4;
-  T* sink = a.sink + (blockIdx.x * 256 + threadIdx.x) % 256;
+  T* sink = a.sink + threadIdx.x;
 
   T wu[6], wvc[6], wh[6];
   for (int m = 0; m < 5; ++m) {
@@ -671,13 +682,10 @@ __device__ __forceinline__ void iface_b(const T* wu, const T* wv, const T* wh, b
 }
 
 template <typename T, bool GSYM>
-__global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
-  __shared__ T s_vct[4][kRowBuf];
-  __shared__ T s_u[4][kRowBuf];
-  __shared__ T s_h[4][kRowBuf];
-  __shared__ T s_n[4][kRowBuf];
+__device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s_vct)[kRowBuf],
+                                             T (*s_u)[kRowBuf], T (*s_h)[kRowBuf], T (*s_n)[kRowBuf]) {
   WaveTile w;
-  if (!decode_wave(a, w)) return;
+  if (!decode_wave(a, L, w)) return;
   const int N = a.g.N;
   T* bvct = s_vct[w.wslot];
   T* bu = s_u[w.wslot];
@@ -692,7 +700,7 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
   hcol = halo_lane ? min(hcol, N - 1) : w.jl;
   const int hslot = halo_lane ? hcol - c0 + 3 : lane + 3;  // others rewrite their own slot
   const bool out_lane = lane < kStripOut && j <= N - 4;
-  T* sink = a.sink + (blockIdx.x * 256 + threadIdx.x) % 256;
+  T* sink = a.sink + threadIdx.x;
 
   // column windows over rows i-3 .. i+2 (slot 5 newest); contravariant v and
   // covariant u only at rows i-1 .. i+2; blended new height over i-4 .. i+2
@@ -848,6 +856,22 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
   }
 }
 
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_march_a(MarchArgs<T> a) {
+  __shared__ T s_v[4][kRowBuf];
+  __shared__ T s_h[4][kRowBuf];
+  march_a_body<T, GSYM>(a, xcd_block(), s_v, s_h);
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
+  __shared__ T s_vct[4][kRowBuf];
+  __shared__ T s_u[4][kRowBuf];
+  __shared__ T s_h[4][kRowBuf];
+  __shared__ T s_n[4][kRowBuf];
+  march_b_body<T, GSYM>(a, xcd_block(), s_vct, s_u, s_h, s_n);
+}
+
 // ---------------------------------------------------------------------------
 // duo-grid halo fill (update_domain pass 1 + cube_rmp, closed index form)
 // ---------------------------------------------------------------------------
@@ -992,8 +1016,7 @@ struct HaloArgs {
 // Scalar halo fill (update_domain pass 1 + cube_rmp, communication.py:404-430,
 // cube.py:64-125) from the plan.
 template <typename T>
-__global__ __launch_bounds__(256) void k_halo_scalar(HaloArgs<T> a) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void halo_scalar_body(const HaloArgs<T>& a, int q) {
   if (q >= a.plan.n) return;
   const uint32_t d = a.plan.dst[q], s0 = a.plan.s0[q], s1 = a.plan.s1[q];
   const T c0 = a.plan.c0[q], c1 = a.plan.c1[q];
@@ -1013,12 +1036,16 @@ __global__ __launch_bounds__(256) void k_halo_scalar(HaloArgs<T> a) {
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_scalar(HaloArgs<T> a) {
+  halo_scalar_body(a, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 // Vector halo fill from the plan: the copied interior values rotated to
 // lat-lon (cube.py:139-140; inner = 1), interpolated, and rotated back with
 // the cell's own rotation (cube.py:147-151; outer = 1).
 template <typename T>
-__global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void halo_vector_body(const HaloArgs<T>& a, int q) {
   const int n = a.plan.n;
   if (q >= n) return;
   const uint32_t d = a.plan.dst[q], s0 = a.plan.s0[q], s1 = a.plan.s1[q];
@@ -1035,6 +1062,95 @@ __global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
   const T p1 = c0 * vl0 + c1 * vl1;
   a.u[d] = R[8] * p0 + R[9] * p1;
   a.v[d] = R[10] * p0 + R[11] * p1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
+  halo_vector_body(a, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// Kernel arguments of one RK stage of one context.
+template <typename T>
+struct StageArgs {
+  EdgeArgs<T> e;
+  MarchArgs<T> ma, mb;
+  HaloArgs<T> hh, huv;
+};
+
+// ---------------------------------------------------------------------------
+// Batched launches: several contexts, of any resolutions, advanced by one
+// launch per kernel class.  Each member's blocks are contiguous; a block finds
+// its member in a block -> member table and reads the member's arguments from
+// a device table of StageArgs.
+// ---------------------------------------------------------------------------
+enum { BK_EDGE = 0, BK_MARCH_A = 1, BK_HALO = 2, BK_MARCH_B = 3, BK_CLASSES = 4 };
+
+template <typename T>
+struct BatchLaunch {
+  const StageArgs<T>* __restrict__ tab;  // [stage][variant][member]
+  const int* __restrict__ nbb;           // substeps of each member
+  const int* __restrict__ blk2m;         // block -> member, this kernel class
+  const int* __restrict__ first;         // first block of each member, this class
+  int count;                             // members in the table
+  int stage, first_sub, parity;          // stage index; substep 0?; substep parity
+};
+
+// Variant 0 / 1 read the caller's input and write the output / the ping-pong
+// field X; variant 2 / 3 read X / the output and write the output / X.  The
+// choice by (nbb - 1 - k) parity makes every member finish its own substep
+// count in its output buffers.
+template <typename T>
+__device__ __forceinline__ const StageArgs<T>& batch_entry(const BatchLaunch<T>& b, int m) {
+  const int e = (b.first_sub ? 0 : 2) + (((b.nbb[m] - 1) & 1) ^ b.parity);
+  return b.tab[(static_cast<size_t>(b.stage) * 4 + e) * b.count + m];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_batch(BatchLaunch<T> b) {
+  const int blk = static_cast<int>(blockIdx.x);
+  const int m = b.blk2m[blk];
+  // by reference: the routing table is indexed per lane, a private copy would
+  // live in scratch memory
+  const EdgeArgs<T>& a = batch_entry(b, m).e;
+  edge_body(a, (blk - b.first[m]) * 256 + static_cast<int>(threadIdx.x));
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_march_a_batch(BatchLaunch<T> b) {
+  __shared__ T s_v[4][kRowBuf];
+  __shared__ T s_h[4][kRowBuf];
+  const int L = xcd_block();
+  const int m = b.blk2m[L];
+  const MarchArgs<T> a = batch_entry(b, m).ma;
+  march_a_body<T, GSYM>(a, L - b.first[m], s_v, s_h);
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_march_b_batch(BatchLaunch<T> b) {
+  __shared__ T s_vct[4][kRowBuf];
+  __shared__ T s_u[4][kRowBuf];
+  __shared__ T s_h[4][kRowBuf];
+  __shared__ T s_n[4][kRowBuf];
+  const int L = xcd_block();
+  const int m = b.blk2m[L];
+  const MarchArgs<T> a = batch_entry(b, m).mb;
+  march_b_body<T, GSYM>(a, L - b.first[m], s_vct, s_u, s_h, s_n);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_scalar_batch(BatchLaunch<T> b) {
+  const int blk = static_cast<int>(blockIdx.x);
+  const int m = b.blk2m[blk];
+  const HaloArgs<T> a = batch_entry(b, m).hh;
+  halo_scalar_body(a, (blk - b.first[m]) * 256 + static_cast<int>(threadIdx.x));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_vector_batch(BatchLaunch<T> b) {
+  const int blk = static_cast<int>(blockIdx.x);
+  const int m = b.blk2m[blk];
+  const HaloArgs<T> a = batch_entry(b, m).huv;
+  halo_vector_body(a, (blk - b.first[m]) * 256 + static_cast<int>(threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1363,10 +1479,25 @@ inline unsigned blocks_for(size_t n, unsigned per = 256) {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// Device storage of one resolution's consts (planes, index tables, halo
+// plan).  Contexts created with sw_create_like share it; it is freed with the
+// last of them.
+struct GridMem {
+  void* planes = nullptr;
+  int* locs = nullptr;
+  void* plan = nullptr;
+  ~GridMem() {
+    if (planes) (void)hipFree(planes);
+    if (locs) (void)hipFree(locs);
+    if (plan) (void)hipFree(plan);
+  }
+};
+
 struct sw_ctx {
   int nx = 0, N = 0, dtype = SW_F64;
   hipStream_t stream = nullptr;
   size_t cells = 0;        // 6 * N * N
+  std::shared_ptr<GridMem> grid;  // owner of planes, locs and plan below
   void* planes = nullptr;  // kNumPlanes T planes
   int* locs = nullptr;     // loc_i, loc_j planes
   void* scratch = nullptr; // 12 T fields: S1, S2 (stage scratch), X, Y (substep ping-pong)
@@ -1392,6 +1523,18 @@ struct sw_ctx {
     hipEvent_t a, b;
   };
   std::vector<ProfRec>* prof = nullptr;  // set only by sw_profile
+};
+
+// Reusable state of sw_batch_run: the device copy of the member tables and
+// its pinned host staging buffer.
+struct sw_batch {
+  void* dev = nullptr;
+  size_t dev_cap = 0;
+  void* host = nullptr;
+  size_t host_cap = 0;
+  hipEvent_t staged = nullptr;   // recorded after the staging copy of the last run
+  hipStream_t stream = nullptr;  // stream of the last run
+  bool used = false;
 };
 
 namespace {
@@ -1478,6 +1621,7 @@ int build_plan(sw_ctx* c) {
   // the rot arrays are addressed k * n + q from p.rot: keep them n apart
   const size_t bytes = 3 * su + 2 * st + kPlanRot * n * sizeof(T) + 256;
   SW_HIP(hipMalloc(&c->plan, bytes));
+  c->grid->plan = c->plan;
   c->plan_n = static_cast<int>(n);
   const HaloPlan<T> p = plan_of<T>(c);
   PlanBuildArgs<T> a{geo_of(c), route(), planes_of<T>(c), const_cast<uint32_t*>(p.dst),
@@ -1506,31 +1650,28 @@ int launch_halo_vector(const sw_ctx* c, T* u, T* v) {
   return SW_OK;
 }
 
-// One RK stage from (h, hp, u, v, up, vp) into (h_out, u_out, v_out), halos
-// filled.  Leaves the averaged edge velocities of this stage in evel.
+// The 12 cube edges, each once, as t * 4 + d with (t, d) before its neighbour.
+int edge_pairs(int ue[12]) {
+  const Route& R = route();
+  int k = 0;
+  for (int t = 0; t < 6; ++t)
+    for (int d = 0; d < 4; ++d)
+      if (t * 4 + d < R.r[t][d][0] * 4 + R.r[t][d][1] && k < 12) ue[k++] = t * 4 + d;
+  if (k != 12) return fail(SW_E_STATE, "routing table does not pair the 24 face edges");
+  return SW_OK;
+}
+
 template <typename T>
-int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, const T* up,
-              const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last) {
+int stage_args(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, const T* up,
+               const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last,
+               StageArgs<T>& A) {
   const Geo g = geo_of(c);
   const Planes<T> P = planes_of<T>(c);
   T* evel = edge_buf<T>(c, 0);
   T* eflux = edge_buf<T>(c, 1);
-  EdgeArgs<T> ea{g, route(), P, h, u, v, evel, eflux, last, {}};
-  {
-    const Route R = route();
-    int k = 0;
-    for (int t = 0; t < 6; ++t)
-      for (int d = 0; d < 4; ++d)
-        if (t * 4 + d < R.r[t][d][0] * 4 + R.r[t][d][1] && k < 12) ea.ue[k++] = t * 4 + d;
-    if (k != 12) return fail(SW_E_STATE, "routing table does not pair the 24 face edges");
-  }
-  int rc = prof_begin(c, K_EDGE);
+  A.e = EdgeArgs<T>{g, route(), P, h, u, v, evel, eflux, last, {}};
+  int rc = edge_pairs(A.e.ue);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), dim3(256), 0,
-                     c->stream, ea);
-  SW_HIP(hipGetLastError());
-  if ((rc = prof_end(c))) return rc;
-
   MarchArgs<T> ma{};
   ma.g = g;
   ma.P = P;
@@ -1544,33 +1685,55 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   ma.S = (c->nx + kStripOut - 1) / kStripOut;
   ma.shared = c->shared;
   ma.gsym = c->gsym;
-  ma.R = c->rows_a;
-  ma.nbands = (c->nx + ma.R - 1) / ma.R;
-  ma.nwaves = 6 * ma.S * ma.nbands;
-  if ((rc = prof_begin(c, K_MARCH_A))) return rc;
   ma.sink = edge_buf<T>(c, 3);
+  A.ma = ma;
+  A.ma.R = c->rows_a;
+  A.ma.nbands = (c->nx + A.ma.R - 1) / A.ma.R;
+  A.ma.nwaves = 6 * ma.S * A.ma.nbands;
+  A.mb = ma;
+  A.mb.R = c->rows_b;
+  A.mb.nbands = (c->nx + A.mb.R - 1) / A.mb.R;
+  A.mb.nwaves = 6 * ma.S * A.mb.nbands;
+  const uint32_t NN = static_cast<uint32_t>(c->N) * static_cast<uint32_t>(c->N);
+  A.hh = HaloArgs<T>{plan_of<T>(c), NN, h_out, 1, nullptr, nullptr};
+  A.huv = HaloArgs<T>{plan_of<T>(c), NN, nullptr, 1, u_out, v_out};
+  return SW_OK;
+}
+
+// One RK stage from (h, hp, u, v, up, vp) into (h_out, u_out, v_out), halos
+// filled.  Leaves the averaged edge velocities of this stage in evel.
+template <typename T>
+int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, const T* up,
+              const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last) {
+  StageArgs<T> A;
+  int rc = stage_args<T>(c, h, hp, u, v, up, vp, h_out, u_out, v_out, dt, beta, last, A);
+  if (rc) return rc;
+  if ((rc = prof_begin(c, K_EDGE))) return rc;
+  hipLaunchKernelGGL(k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), dim3(256), 0,
+                     c->stream, A.e);
+  SW_HIP(hipGetLastError());
+  if ((rc = prof_end(c))) return rc;
+  if ((rc = prof_begin(c, K_MARCH_A))) return rc;
   if (c->gsym)
-    hipLaunchKernelGGL((k_march_a<T, true>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+    hipLaunchKernelGGL((k_march_a<T, true>), dim3((A.ma.nwaves + 3) / 4), dim3(256), 0, c->stream, A.ma);
   else
-    hipLaunchKernelGGL((k_march_a<T, false>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+    hipLaunchKernelGGL((k_march_a<T, false>), dim3((A.ma.nwaves + 3) / 4), dim3(256), 0, c->stream, A.ma);
   SW_HIP(hipGetLastError());
   if ((rc = prof_end(c))) return rc;
   if ((rc = prof_begin(c, K_HALO_H))) return rc;
-  rc = launch_halo_scalar<T>(c, h_out, 1);
-  if (rc) return rc;
+  hipLaunchKernelGGL(k_halo_scalar<T>, dim3(blocks_for(c->plan_n)), dim3(256), 0, c->stream, A.hh);
+  SW_HIP(hipGetLastError());
   if ((rc = prof_end(c))) return rc;
-  ma.R = c->rows_b;
-  ma.nbands = (c->nx + ma.R - 1) / ma.R;
-  ma.nwaves = 6 * ma.S * ma.nbands;
   if ((rc = prof_begin(c, K_MARCH_B))) return rc;
   if (c->gsym)
-    hipLaunchKernelGGL((k_march_b<T, true>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+    hipLaunchKernelGGL((k_march_b<T, true>), dim3((A.mb.nwaves + 3) / 4), dim3(256), 0, c->stream, A.mb);
   else
-    hipLaunchKernelGGL((k_march_b<T, false>), dim3((ma.nwaves + 3) / 4), dim3(256), 0, c->stream, ma);
+    hipLaunchKernelGGL((k_march_b<T, false>), dim3((A.mb.nwaves + 3) / 4), dim3(256), 0, c->stream, A.mb);
   SW_HIP(hipGetLastError());
   if ((rc = prof_end(c))) return rc;
   if ((rc = prof_begin(c, K_HALO_UV))) return rc;
-  if ((rc = launch_halo_vector<T>(c, u_out, v_out))) return rc;
+  hipLaunchKernelGGL(k_halo_vector<T>, dim3(blocks_for(c->plan_n)), dim3(256), 0, c->stream, A.huv);
+  SW_HIP(hipGetLastError());
   return prof_end(c);
 }
 
@@ -1737,15 +1900,27 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   return SW_OK;
 }
 
+// Per-context workspace: stage scratch, substep ping-pong fields, edge lines.
 template <typename T>
-int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
+int alloc_workspace(sw_ctx* c) {
   const size_t n = c->cells;
-  SW_HIP(hipMalloc(&c->planes, kNumPlanes * n * sizeof(T)));
-  SW_HIP(hipMalloc(&c->locs, 2 * n * sizeof(int)));
   SW_HIP(hipMalloc(&c->scratch, 12 * n * sizeof(T)));
   SW_HIP(hipMalloc(&c->edges, 4 * edge_stride(c) * sizeof(T)));
   SW_HIP(hipMemsetAsync(c->scratch, 0, 12 * n * sizeof(T), c->stream));
   SW_HIP(hipMemsetAsync(c->edges, 0, 4 * edge_stride(c) * sizeof(T), c->stream));
+  return SW_OK;
+}
+
+template <typename T>
+int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
+  const size_t n = c->cells;
+  c->grid = std::make_shared<GridMem>();
+  SW_HIP(hipMalloc(&c->planes, kNumPlanes * n * sizeof(T)));
+  c->grid->planes = c->planes;
+  SW_HIP(hipMalloc(&c->locs, 2 * n * sizeof(int)));
+  c->grid->locs = c->locs;
+  int rc = alloc_workspace<T>(c);
+  if (rc) return rc;
   T* p = static_cast<T*>(c->planes);
   RepackArgs<T> a{};
   a.n = n;
@@ -1835,12 +2010,10 @@ void free_ctx(sw_ctx* c) {
   for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
   c->graphs.clear();
   if (c->capture) (void)hipStreamDestroy(c->capture);
-  if (c->planes) (void)hipFree(c->planes);
-  if (c->locs) (void)hipFree(c->locs);
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->edges) (void)hipFree(c->edges);
   if (c->ibuf) (void)hipFree(c->ibuf);
-  if (c->plan) (void)hipFree(c->plan);
+  c->grid.reset();
   delete c;
 }
 
@@ -1859,6 +2032,224 @@ int run_integrals(sw_ctx* c, const void* h, const void* u, const void* v, double
   SW_HIP(hipMemcpyAsync(out, c->ibuf + 4 * kIntegralBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost,
                         c->stream));
   SW_HIP(hipStreamSynchronize(c->stream));
+  return SW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched runs
+// ---------------------------------------------------------------------------
+constexpr int kBatchMax = 4096;
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// nbb substeps for every member, all members advanced by the same launches.
+// Members are ordered by decreasing nbb, so the members still stepping at
+// substep k are a prefix of the tables and the grid of a launch covers
+// exactly their blocks.
+template <typename T>
+int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, hipStream_t stream) {
+  // ---- validation
+  std::vector<const void*> outs;
+  std::vector<const void*> ins;
+  bool gsym = true;
+  for (int i = 0; i < count; ++i) {
+    const sw_member& m = mem[i];
+    if (!m.ctx) return fail(SW_E_ARG, "member " + std::to_string(i) + ": ctx is null");
+    if (m.ctx->dtype != (std::is_same<T, double>::value ? SW_F64 : SW_F32))
+      return fail(SW_E_ARG, "members must share one dtype");
+    int rc = check_ptrs(m.in, 6, "in");
+    if (!rc) rc = check_ptrs(const_cast<const void* const*>(m.out), 13, "out");
+    if (!rc) rc = check_disjoint(m.ctx, m.in, m.out);
+    if (rc) return fail(rc, "member " + std::to_string(i) + ": " + sw_last_error());
+    if (m.nbb < 0) return fail(SW_E_ARG, "member " + std::to_string(i) + ": nbb < 0");
+    if (!(m.dtb == m.dtb) || m.dtb > 1e300 || m.dtb < -1e300)
+      return fail(SW_E_ARG, "member " + std::to_string(i) + ": dtb must be finite");
+    for (int k = 0; k < 13; ++k) outs.push_back(m.out[k]);
+    for (int k = 0; k < 6; ++k) ins.push_back(m.in[k]);
+    gsym = gsym && m.ctx->gsym;
+  }
+  {
+    std::vector<const sw_ctx*> ctxs;
+    for (int i = 0; i < count; ++i) ctxs.push_back(mem[i].ctx);
+    std::sort(ctxs.begin(), ctxs.end());
+    if (std::adjacent_find(ctxs.begin(), ctxs.end()) != ctxs.end())
+      return fail(SW_E_ARG, "each member needs its own context (sw_create_like shares a grid)");
+    std::sort(outs.begin(), outs.end());
+    if (std::adjacent_find(outs.begin(), outs.end()) != outs.end())
+      return fail(SW_E_ARG, "output buffers of the members alias each other");
+    for (const void* p : ins)
+      if (std::binary_search(outs.begin(), outs.end(), p))
+        return fail(SW_E_ARG, "an output buffer aliases an input of another member");
+  }
+
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return mem[x].nbb > mem[y].nbb; });
+  int K = 0;
+  while (K < count && mem[order[K]].nbb > 0) ++K;
+  for (int i = K; i < count; ++i) {  // nbb = 0: the reference's empty loop returns the input
+    const sw_member& m = mem[order[i]];
+    const size_t bytes = m.ctx->cells * sizeof(T);
+    SW_HIP(hipMemcpyAsync(m.out[0], m.in[0], bytes, hipMemcpyDeviceToDevice, stream));
+    SW_HIP(hipMemcpyAsync(m.out[1], m.in[2], bytes, hipMemcpyDeviceToDevice, stream));
+    SW_HIP(hipMemcpyAsync(m.out[2], m.in[3], bytes, hipMemcpyDeviceToDevice, stream));
+  }
+  if (K == 0) return SW_OK;
+
+  // ---- tables: StageArgs [stage][variant][member], nbb, first / blk2m per class
+  const size_t ntab = static_cast<size_t>(rk) * 4 * K;
+  std::vector<StageArgs<T>> tab(ntab);
+  std::vector<int> nbb(K);
+  std::vector<int> first[BK_CLASSES];
+  for (auto& f : first) f.assign(K + 1, 0);
+  for (int q = 0; q < K; ++q) {
+    const sw_member& m = mem[order[q]];
+    const sw_ctx* c = m.ctx;
+    nbb[q] = m.nbb;
+    const T* in[6];
+    for (int k = 0; k < 6; ++k) in[k] = static_cast<const T*>(m.in[k]);
+    T* fin[3] = {static_cast<T*>(m.out[0]), static_cast<T*>(m.out[1]), static_cast<T*>(m.out[2])};
+    T* X[3] = {scratch_field<T>(c, 6), scratch_field<T>(c, 7), scratch_field<T>(c, 8)};
+    T* S1[3] = {scratch_field<T>(c, 0), scratch_field<T>(c, 1), scratch_field<T>(c, 2)};
+    T* S2[3] = {scratch_field<T>(c, 3), scratch_field<T>(c, 4), scratch_field<T>(c, 5)};
+    for (int e = 0; e < 4; ++e) {
+      // source (h, hp, u, v, up, vp) and destination of the substep
+      const T* src[6];
+      if (e < 2) {
+        for (int k = 0; k < 6; ++k) src[k] = in[k];
+      } else {
+        T* const* a = (e == 2) ? X : fin;
+        const T* f6[6] = {a[0], a[0], a[1], a[2], a[1], a[2]};
+        for (int k = 0; k < 6; ++k) src[k] = f6[k];
+      }
+      T* const* dst = (e == 0 || e == 2) ? fin : X;
+      const double dtb = m.dtb;
+      int rc;
+      StageArgs<T>* row = &tab[0];
+      auto at = [&](int st) -> StageArgs<T>& { return row[(static_cast<size_t>(st) * 4 + e) * K + q]; };
+      if (rk == 2) {
+        rc = stage_args<T>(c, src[0], src[1], src[2], src[3], src[4], src[5], S1[0], S1[1], S1[2],
+                           dtb / 2.0, 0.0, 0, at(0));
+        if (!rc) rc = stage_args<T>(c, S1[0], src[1], S1[1], S1[2], src[4], src[5], dst[0], dst[1],
+                                    dst[2], dtb, 0.0, 1, at(1));
+      } else {
+        rc = stage_args<T>(c, src[0], src[1], src[2], src[3], src[4], src[5], S1[0], S1[1], S1[2],
+                           dtb / 3.0, 1.0, 0, at(0));
+        if (!rc) rc = stage_args<T>(c, S1[0], src[1], S1[1], S1[2], src[4], src[5], S2[0], S2[1],
+                                    S2[2], dtb / 2.0, 1.0, 0, at(1));
+        if (!rc) rc = stage_args<T>(c, S2[0], src[1], S2[1], S2[2], src[4], src[5], dst[0], dst[1],
+                                    dst[2], dtb, 1.0, 1, at(2));
+      }
+      if (rc) return rc;
+    }
+    const StageArgs<T>& A = tab[q];
+    const int blocks[BK_CLASSES] = {static_cast<int>(blocks_for(24 * static_cast<size_t>(c->nx))),
+                                    (A.ma.nwaves + 3) / 4, static_cast<int>(blocks_for(c->plan_n)),
+                                    (A.mb.nwaves + 3) / 4};
+    for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
+  }
+  // byte layout of the device copy
+  size_t off = 0;
+  const size_t o_tab = off;
+  off = align256(off + ntab * sizeof(StageArgs<T>));
+  const size_t o_nbb = off;
+  off = align256(off + K * sizeof(int));
+  size_t o_first[BK_CLASSES], o_blk[BK_CLASSES];
+  for (int k = 0; k < BK_CLASSES; ++k) {
+    o_first[k] = off;
+    off = align256(off + (K + 1) * sizeof(int));
+    o_blk[k] = off;
+    off = align256(off + static_cast<size_t>(first[k][K]) * sizeof(int));
+  }
+  const size_t bytes = off;
+
+  // ---- stage into pinned memory and copy (the previous run's copy must be done)
+  if (b->staged) SW_HIP(hipEventSynchronize(b->staged));
+  if (b->host_cap < bytes) {
+    if (b->host) SW_HIP(hipHostFree(b->host));
+    b->host = nullptr;
+    SW_HIP(hipHostMalloc(&b->host, bytes, hipHostMallocDefault));
+    b->host_cap = bytes;
+  }
+  if (b->dev_cap < bytes) {
+    if (b->used && b->stream) SW_HIP(hipStreamSynchronize(b->stream));  // kernels may still read it
+    if (b->used && b->stream != stream) SW_HIP(hipStreamSynchronize(stream));
+    if (b->dev) SW_HIP(hipFree(b->dev));
+    b->dev = nullptr;
+    SW_HIP(hipMalloc(&b->dev, bytes));
+    b->dev_cap = bytes;
+  } else if (b->used && b->stream && b->stream != stream) {
+    // the previous run's launches on another stream still read the tables
+    SW_HIP(hipStreamSynchronize(b->stream));
+  }
+  char* h = static_cast<char*>(b->host);
+  std::memcpy(h + o_tab, tab.data(), ntab * sizeof(StageArgs<T>));
+  std::memcpy(h + o_nbb, nbb.data(), K * sizeof(int));
+  for (int k = 0; k < BK_CLASSES; ++k) {
+    std::memcpy(h + o_first[k], first[k].data(), (K + 1) * sizeof(int));
+    int* blk = reinterpret_cast<int*>(h + o_blk[k]);
+    for (int q = 0; q < K; ++q)
+      for (int x = first[k][q]; x < first[k][q + 1]; ++x) blk[x] = q;
+  }
+  SW_HIP(hipMemcpyAsync(b->dev, b->host, bytes, hipMemcpyHostToDevice, stream));
+  if (!b->staged) SW_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+  SW_HIP(hipEventRecord(b->staged, stream));
+  b->stream = stream;
+  b->used = true;
+
+  // ---- launches
+  const char* d = static_cast<const char*>(b->dev);
+  BatchLaunch<T> L[BK_CLASSES];
+  for (int k = 0; k < BK_CLASSES; ++k) {
+    L[k].tab = reinterpret_cast<const StageArgs<T>*>(d + o_tab);
+    L[k].nbb = reinterpret_cast<const int*>(d + o_nbb);
+    L[k].first = reinterpret_cast<const int*>(d + o_first[k]);
+    L[k].blk2m = reinterpret_cast<const int*>(d + o_blk[k]);
+    L[k].count = K;
+  }
+  int active = K;
+  for (int k = 0; k < nbb[0]; ++k) {
+    while (active > 0 && nbb[active - 1] <= k) --active;
+    for (int st = 0; st < rk; ++st) {
+      for (auto& l : L) {
+        l.stage = st;
+        l.first_sub = (k == 0);
+        l.parity = k & 1;
+      }
+      hipLaunchKernelGGL(k_edge_batch<T>, dim3(first[BK_EDGE][active]), dim3(256), 0, stream, L[BK_EDGE]);
+      if (gsym)
+        hipLaunchKernelGGL((k_march_a_batch<T, true>), dim3(first[BK_MARCH_A][active]), dim3(256), 0, stream,
+                           L[BK_MARCH_A]);
+      else
+        hipLaunchKernelGGL((k_march_a_batch<T, false>), dim3(first[BK_MARCH_A][active]), dim3(256), 0, stream,
+                           L[BK_MARCH_A]);
+      hipLaunchKernelGGL(k_halo_scalar_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, stream,
+                         L[BK_HALO]);
+      if (gsym)
+        hipLaunchKernelGGL((k_march_b_batch<T, true>), dim3(first[BK_MARCH_B][active]), dim3(256), 0, stream,
+                           L[BK_MARCH_B]);
+      else
+        hipLaunchKernelGGL((k_march_b_batch<T, false>), dim3(first[BK_MARCH_B][active]), dim3(256), 0, stream,
+                           L[BK_MARCH_B]);
+      hipLaunchKernelGGL(k_halo_vector_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, stream,
+                         L[BK_HALO]);
+      SW_HIP(hipGetLastError());
+    }
+  }
+  // ---- diagnostics of each member from the inputs of its last stage
+  if (!(flags & SW_SKIP_DIAG)) {
+    for (int q = 0; q < K; ++q) {
+      const sw_member& m = mem[order[q]];
+      sw_ctx* c = m.ctx;
+      const int ls = rk == 2 ? 0 : 3;
+      hipStream_t keep = c->stream;
+      c->stream = stream;
+      const int rc = run_diag<T>(c, scratch_field<T>(c, ls), scratch_field<T>(c, ls + 1),
+                                 scratch_field<T>(c, ls + 2), m.out);
+      c->stream = keep;
+      if (rc) return rc;
+    }
+  }
   return SW_OK;
 }
 
@@ -1896,6 +2287,41 @@ int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CON
 void sw_destroy(sw_ctx* ctx) {
   if (ctx) (void)hipStreamSynchronize(ctx->stream);
   free_ctx(ctx);
+}
+
+int sw_create_like(sw_ctx** out, const sw_ctx* base) {
+  if (!out) return fail(SW_E_ARG, "out is null");
+  *out = nullptr;
+  if (!base) return fail(SW_E_ARG, "base is null");
+  sw_ctx* c = new sw_ctx();
+  c->nx = base->nx;
+  c->N = base->N;
+  c->dtype = base->dtype;
+  c->stream = base->stream;
+  c->cells = base->cells;
+  c->grid = base->grid;
+  c->planes = base->planes;
+  c->locs = base->locs;
+  c->plan = base->plan;
+  c->plan_n = base->plan_n;
+  c->dzx_s = base->dzx_s;
+  c->dzy_s = base->dzy_s;
+  c->dz_uniform = base->dz_uniform;
+  c->shared = base->shared;
+  c->gsym = base->gsym;
+  c->rows_a = base->rows_a;
+  c->rows_b = base->rows_b;
+  int rc = c->dtype == SW_F64 ? alloc_workspace<double>(c) : alloc_workspace<float>(c);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(SW_E_HIP, std::string("sw_create_like: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return SW_OK;
 }
 
 int sw_set_stream(sw_ctx* ctx, void* stream) {
@@ -2040,6 +2466,35 @@ int sw_info(const sw_ctx* ctx, int* nx, int* dtype) {
   if (nx) *nx = ctx->nx;
   if (dtype) *dtype = ctx->dtype;
   return SW_OK;
+}
+
+int sw_batch_create(sw_batch** out) {
+  if (!out) return fail(SW_E_ARG, "out is null");
+  *out = new sw_batch();
+  return SW_OK;
+}
+
+void sw_batch_destroy(sw_batch* b) {
+  if (!b) return;
+  if (b->used && b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->staged) {
+    (void)hipEventSynchronize(b->staged);
+    (void)hipEventDestroy(b->staged);
+  }
+  if (b->dev) (void)hipFree(b->dev);
+  if (b->host) (void)hipHostFree(b->host);
+  delete b;
+}
+
+int sw_batch_run(sw_batch* b, const sw_member* members, int count, int rk, int flags, void* stream) {
+  if (!b) return fail(SW_E_ARG, "batch is null");
+  if (!members) return fail(SW_E_ARG, "members is null");
+  if (count < 1 || count > kBatchMax) return fail(SW_E_ARG, "count must be in [1, 4096]");
+  if (rk != 2 && rk != 3) return fail(SW_E_ARG, "rk must be 2 or 3");
+  if (!members[0].ctx) return fail(SW_E_ARG, "member 0: ctx is null");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return members[0].ctx->dtype == SW_F64 ? run_batch<double>(b, members, count, rk, flags, s)
+                                         : run_batch<float>(b, members, count, rk, flags, s);
 }
 
 const char* sw_last_error(void) { return g_err.c_str(); }

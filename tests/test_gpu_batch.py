@@ -1,0 +1,104 @@
+"""Batched runs (sw_batch_run): each member equals the same run alone (GPU).
+
+Members of different resolutions, substep counts and lengths are advanced by
+the same launches; every output of every member must be bit-identical to
+sw_rk2 / sw_rk3 on that member's context alone, and (FP64) to the oracle.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import barotr_np as B
+from tests._util import grid_consts, random_core
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _members(torch, dtype, spec, seed=0):
+    """spec: list of (nx, nbb, dtb); two entries of one nx share a grid."""
+    from licom4py_gpu_amd import ensemble
+    from licom4py_gpu_amd.barotr import SWContext
+    base = {}
+    out = []
+    for k, (nx, nbb, dtb) in enumerate(spec):
+        _, consts = grid_consts(nx)
+        if nx in base:
+            ctx = base[nx].like()
+        else:
+            ctx = base[nx] = SWContext(consts, dtype)
+        core_np = random_core(nx, seed=seed + k, distinct_p=False)
+        core = [torch.as_tensor(x).cuda().to(dtype) for x in core_np]
+        out.append(ensemble.Member(ctx, core, nbb, dtb, tag={"consts": consts, "core": core_np}))
+    return out
+
+
+def _check_alone(torch, members, rk=2, diagnostics=True):
+    for k, m in enumerate(members):
+        alone = m.ctx.rk(m.core6, m.nbb, m.dtb, rk=rk, diagnostics=diagnostics, graph=False)
+        n = 13 if (diagnostics and m.nbb > 0) else 3
+        for i in range(n):
+            a, b = m.out[i].cpu().numpy(), alone[i].cpu().numpy()
+            assert np.array_equal(a, b), f"member {k} (C{m.ctx.nx}, nbb {m.nbb}) output {i} differs"
+
+
+SPEC = [(8, 5, 40.0), (13, 3, 25.0), (16, 0, 30.0), (16, 4, 31.0), (24, 1, 12.5), (10, 4, 33.0)]
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+def test_batch_fp64_matches_alone_and_oracle(rk):
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    members = _members(torch, torch.float64, SPEC)
+    ensemble.Batch().run(members, rk=rk)
+    _check_alone(torch, members, rk=rk)
+    # the C8 member against the oracle
+    m = members[0]
+    core = m.tag["core"]
+    st = list(core) + [np.zeros_like(core[0])] * 17
+    fn = B.barotr_rk2_core if rk == 2 else B.barotr_rk3_core
+    want = fn(tuple(st), m.tag["consts"], m.nbb, m.dtb)
+    assert np.array_equal(m.out[0].cpu().numpy(), want[0])
+    assert np.array_equal(m.out[1].cpu().numpy(), want[2])
+    assert np.array_equal(m.out[2].cpu().numpy(), want[3])
+
+
+def test_batch_fp32_matches_alone():
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    members = _members(torch, torch.float32, SPEC, seed=7)
+    ensemble.Batch().run(members)
+    _check_alone(torch, members)
+
+
+def test_batch_object_reused_with_other_members():
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    b = ensemble.Batch()
+    first = _members(torch, torch.float64, SPEC, seed=3)
+    b.run(first, diagnostics=False)
+    second = _members(torch, torch.float64, [(12, 6, 20.0), (9, 2, 44.0)], seed=5)
+    b.run(second, diagnostics=False)
+    _check_alone(torch, first, diagnostics=False)
+    _check_alone(torch, second, diagnostics=False)
+
+
+def test_batch_rejects_shared_context_and_mixed_dtypes():
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    members = _members(torch, torch.float64, [(8, 2, 10.0), (8, 2, 10.0)])
+    members[1].ctx = members[0].ctx
+    with pytest.raises(RuntimeError, match="own context"):
+        ensemble.Batch().run(members)
+    m64 = _members(torch, torch.float64, [(8, 2, 10.0)])
+    m32 = _members(torch, torch.float32, [(9, 2, 10.0)])
+    with pytest.raises(RuntimeError, match="dtype"):
+        ensemble.Batch().run(m64 + m32)
