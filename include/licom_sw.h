@@ -76,6 +76,7 @@ int sw_set_stream(sw_ctx* ctx, void* stream);
  *   in[6]  = h0, h0p, ub, vb, ubp, vbp          (read only; may alias)
  *   out[13]= h0, ub, vb, celerity_x, celerity_y, ub_ct, vb_ct, ub_cx, ub_cy,
  *            vb_cx, vb_cy, advx, advy          (distinct from in[])
+ * With SW_SKIP_DIAG, out[3..12] are not written and may be null.
  * The reference returns h0p = h0, ubp = ub, vbp = vb and passes the coupling
  * fields through; callers alias those.  nbb = 0 copies in -> out (h0, ub, vb)
  * and leaves the diagnostics untouched, as the reference's empty loop does. */

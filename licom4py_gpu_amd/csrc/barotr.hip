@@ -1763,10 +1763,10 @@ int check_ptrs(const void* const* p, int n, const char* what) {
 int check_disjoint(const sw_ctx* c, const void* const in[6], void* const out[13]) {
   for (int k = 0; k < 13; ++k)
     for (int m = 0; m < 6; ++m)
-      if (out[k] == in[m]) return fail(SW_E_ARG, "output " + std::to_string(k) + " aliases an input");
+      if (out[k] && out[k] == in[m]) return fail(SW_E_ARG, "output " + std::to_string(k) + " aliases an input");
   for (int k = 0; k < 13; ++k)
     for (int m = k + 1; m < 13; ++m)
-      if (out[k] == out[m]) return fail(SW_E_ARG, "outputs alias each other");
+      if (out[k] && out[k] == out[m]) return fail(SW_E_ARG, "outputs alias each other");
   (void)c;
   return SW_OK;
 }
@@ -1842,7 +1842,8 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
            int rk) {
   int rc = check_ptrs(in, 6, "in");
   if (rc) return rc;
-  rc = check_ptrs(const_cast<const void* const*>(out), 13, "out");
+  // the 10 diagnostic outputs may be null when they are not written
+  rc = check_ptrs(const_cast<const void* const*>(out), (flags & SW_SKIP_DIAG) ? 3 : 13, "out");
   if (rc) return rc;
   rc = check_disjoint(c, in, out);
   if (rc) return rc;
@@ -2058,13 +2059,14 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     if (m.ctx->dtype != (std::is_same<T, double>::value ? SW_F64 : SW_F32))
       return fail(SW_E_ARG, "members must share one dtype");
     int rc = check_ptrs(m.in, 6, "in");
-    if (!rc) rc = check_ptrs(const_cast<const void* const*>(m.out), 13, "out");
+    if (!rc) rc = check_ptrs(const_cast<const void* const*>(m.out), (flags & SW_SKIP_DIAG) ? 3 : 13, "out");
     if (!rc) rc = check_disjoint(m.ctx, m.in, m.out);
     if (rc) return fail(rc, "member " + std::to_string(i) + ": " + sw_last_error());
     if (m.nbb < 0) return fail(SW_E_ARG, "member " + std::to_string(i) + ": nbb < 0");
     if (!(m.dtb == m.dtb) || m.dtb > 1e300 || m.dtb < -1e300)
       return fail(SW_E_ARG, "member " + std::to_string(i) + ": dtb must be finite");
-    for (int k = 0; k < 13; ++k) outs.push_back(m.out[k]);
+    for (int k = 0; k < 13; ++k)
+      if (m.out[k]) outs.push_back(m.out[k]);
     for (int k = 0; k < 6; ++k) ins.push_back(m.in[k]);
     gsym = gsym && m.ctx->gsym;
   }

@@ -60,7 +60,8 @@ class Batch:
 
     def run(self, members: Sequence[Member], rk: int = 2, diagnostics: bool = True):
         """Advance every member by its own nbb substeps; fills and returns each
-        member's 13 outputs (h0, ub, vb, 10 diagnostics)."""
+        member's outputs (h0, ub, vb, then the 10 diagnostics unless
+        ``diagnostics`` is false, when 3 output tensors suffice)."""
         if not members:
             return []
         arr = (_hip.SWMember * len(members))()
@@ -69,13 +70,15 @@ class Batch:
         for k, m in enumerate(members):
             ins = [m.ctx._check_field(x, barotr.STATE_KEYS[i]) for i, x in enumerate(m.core6)]
             if m.out is None:
-                m.out = m.ctx.empty(13)
+                m.out = m.ctx.empty(13 if diagnostics else 3)
+            if diagnostics and len(m.out) < 13:
+                raise ValueError("diagnostics need 13 output tensors per member")
             keep.append(ins)
             arr[k].ctx = m.ctx._h.value
             for i in range(6):
                 arr[k].inp[i] = ins[i].data_ptr()
-            for i in range(13):
-                arr[k].out[i] = m.out[i].data_ptr()
+            for i in range(13):  # diagnostic slots stay null when not written
+                arr[k].out[i] = m.out[i].data_ptr() if i < len(m.out) else None
             arr[k].nbb = int(m.nbb)
             arr[k].dtb = float(m.dtb)
         stream = torch.cuda.current_stream(dev).cuda_stream
