@@ -57,7 +57,7 @@ def parse():
                    help="keep the default band height instead of timing the candidates")
     p.add_argument("--profile-substeps", type=int, default=20,
                    help="substeps of the eager event-timed pass (0 disables)")
-    p.add_argument("--cpu-substeps", type=int, default=3,
+    p.add_argument("--cpu-substeps", type=int, default=8,
                    help="substeps of the CPU oracle sample (0 disables)")
     p.add_argument("--cpu-nx", type=int, default=None, help="resolution of the CPU sample (default --nx)")
     return p.parse_args()

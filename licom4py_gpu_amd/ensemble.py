@@ -90,10 +90,14 @@ class Batch:
 
 class GridPool:
     """Per (resolution, dtype): the grid, its device consts and a base context
-    whose const planes later members share."""
+    whose const planes later members share.  With ``tune`` the base context's
+    band heights are picked by the autotuner (on the seeded field of that
+    resolution, cached per device) and inherited by every member context."""
 
-    def __init__(self, device=None):
+    def __init__(self, device=None, tune: bool = False):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.tune = tune
+        self.tuning: Dict[tuple, dict] = {}
         self._consts: Dict[int, tuple] = {}
         self._base: Dict[tuple, "barotr.SWContext"] = {}
 
@@ -106,8 +110,17 @@ class GridPool:
     def context(self, nx: int, dtype: torch.dtype) -> "barotr.SWContext":
         key = (nx, dtype)
         if key not in self._base:
-            self._base[key] = barotr.SWContext(self.consts(nx)[1], dtype)
-            return self._base[key]
+            ctx = barotr.SWContext(self.consts(nx)[1], dtype)
+            if self.tune:
+                from . import autotune
+                g = self.consts(nx)[0]
+                h, u, v, sp = initial.seeded_balanced(g)
+                core = [torch.as_tensor(x).to(device=self.device, dtype=dtype)
+                        for x in initial.state_tuple(h, u, v)[:6]]
+                r = autotune.tune(ctx, core, initial.courant_dt(g, h, sp))
+                self.tuning[key] = {"rows_a": r["rows_a"], "rows_b": r["rows_b"]}
+            self._base[key] = ctx
+            return ctx
         return self._base[key].like()
 
 
@@ -137,11 +150,12 @@ def main(argv=None) -> int:
     p.add_argument("--dtype", choices=("f64", "f32"), default="f64")
     p.add_argument("--seed", type=int, default=20261203)
     p.add_argument("--reps", type=int, default=2)
+    p.add_argument("--no-autotune", action="store_true")
     a = p.parse_args(argv)
     tdt = torch.float64 if a.dtype == "f64" else torch.float32
     rng = np.random.default_rng(a.seed)
     res = [int(x) for x in rng.choice(a.res, size=a.members)]
-    pool = GridPool()
+    pool = GridPool(tune=not a.no_autotune)
     members = seeded_members(pool, res, tdt, a.substeps, seed=a.seed)
     batch = Batch()
     batch.run(members, diagnostics=False)  # warm-up
@@ -170,6 +184,7 @@ def main(argv=None) -> int:
         "sequential_cell_substeps_per_s": cells / min(ts),
         "speedup": min(ts) / min(tb),
         "bit_identical_to_alone": bool(same),
+        "tuning": {f"C{k[0]}": v for k, v in pool.tuning.items()},
     }
     print(json.dumps(line), flush=True)
     return 0 if same else 1

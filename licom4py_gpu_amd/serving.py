@@ -220,6 +220,36 @@ def latency_stats(done: Sequence[Request]) -> dict:
             "admission_wait_p99_s": float(np.percentile(wait, 99))}
 
 
+def one_at_a_time(requests: Sequence[Request], pool: "ensemble.GridPool", force_fp64: bool = False):
+    """The plain path as the comparison: every request run alone (sw_rk2 with
+    recorded substep graphs) in arrival order.  Service times are measured;
+    latencies follow from a first-come-first-served single queue."""
+    requests = sorted(requests, key=lambda r: r.arrival)
+    service = []
+    ctxs: Dict[Tuple[int, torch.dtype], "barotr.SWContext"] = {}
+    for r in requests:
+        dtype = torch.float64 if force_fp64 else r.dtype
+        key = (r.nx, dtype)
+        if key not in ctxs:
+            ctxs[key] = pool.context(r.nx, dtype)
+        h, u, v = (x.to(device=pool.device, dtype=dtype) for x in r.init)
+        torch.cuda.synchronize(pool.device)
+        t0 = time.perf_counter()
+        ctxs[key].rk((h, h, u, v, u, v), r.steps, r.dt, diagnostics=False, graph=True)
+        torch.cuda.synchronize(pool.device)
+        service.append(time.perf_counter() - t0)
+    end = 0.0
+    lat = []
+    for r, s in zip(requests, service):
+        end = max(end, r.arrival) + s
+        lat.append(end - r.arrival)
+    lat = np.array(lat)
+    work = sum(6 * r.nx * r.nx * r.steps for r in requests)
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "makespan_s": float(end), "busy_s": float(sum(service)),
+            "throughput_cell_substeps_per_s": work / sum(service)}
+
+
 def seeded_trace(n: int, rate: float, resolutions: Sequence[int], steps_range=(50, 5000),
                  fp32_share: float = 0.5, seed: int = 7, device=None, keep_result: bool = False):
     """Poisson arrivals of the seeded balanced recipe: each request scales one
@@ -262,6 +292,8 @@ def main(argv=None) -> int:
     p.add_argument("--budget-gb", type=float, default=200.0)
     p.add_argument("--chunk", type=int, default=100)
     p.add_argument("--seed", type=int, default=7)
+    p.add_argument("--no-baseline", action="store_true",
+                   help="skip the one-request-at-a-time comparison runs")
     a = p.parse_args(argv)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -293,6 +325,9 @@ def main(argv=None) -> int:
         "peak_hbm_gb": srv.peak_used / 1e9, "budget_gb": a.budget_gb,
         "finite": bool(all(math.isfinite(r.checksum) for r in done)),
     }
+    if not a.no_baseline:
+        line["one_at_a_time"] = one_at_a_time(reqs, srv.pool)
+        line["one_at_a_time_fp64"] = one_at_a_time(reqs, srv.pool, force_fp64=True)
     print(json.dumps(line), flush=True)
     return 0 if line["finite"] else 1
 
