@@ -9,8 +9,11 @@ substeps / wall time (cell-substeps/s).
 
 The JSON line also carries
   roofline       the dominant kernel's algorithmic HBM bytes per launch over
-                 its average launch time, measured here with hipEvents on the
-                 launch stream (eager pass of the same workload),
+                 its average launch time, measured here with start / stop
+                 hipEvents attached to each dispatch on the launch stream
+                 (hipExtLaunchKernelGGL) in an eager pass of the same
+                 workload, kernels in their step order, and its HBM traffic
+                 per launch from the committed rocprofv3 counter summary,
   roofline_path  the whole step against 296 B per FP64 cell-substep (148 B for
                  FP32) at 8 TB/s,
   cpu_baseline   the NumPy FP64 oracle on one host core over a bounded sample
@@ -55,8 +58,8 @@ def parse():
     p.add_argument("--no-graph", action="store_true", help="launch every substep eagerly")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default band height instead of timing the candidates")
-    p.add_argument("--profile-substeps", type=int, default=20,
-                   help="substeps of the eager event-timed pass (0 disables)")
+    p.add_argument("--profile-substeps", type=int, default=10,
+                   help="substeps of the event-timed eager pass (0 disables)")
     p.add_argument("--cpu-substeps", type=int, default=8,
                    help="substeps of the CPU oracle sample (0 disables)")
     p.add_argument("--cpu-nx", type=int, default=None, help="resolution of the CPU sample (default --nx)")
@@ -162,6 +165,7 @@ def main() -> int:
                        "us_avg": round(ms / n * 1e3, 3) if n else None}
                    for k, (ms, n) in prof.items()}
         dom = max(("march_a", "march_b"), key=lambda k: prof[k][0])
+        kernels["timing"] = "start/stop hipEvents on each dispatch (hipExtLaunchKernelGGL), eager pass"
         ms, n = prof[dom]
         avg_s = ms / n * 1e-3
         esize = 8 if args.dtype == "f64" else 4

@@ -107,14 +107,16 @@ int sw_sync(sw_ctx* ctx);
  * transform 0=ID / 1=FLIP), directions E=0, W=1, N=2, S=3. */
 int sw_routing_table(int32_t out[6 * 4 * 3]);
 
-/* Eager RK2 run of nbb substeps (as sw_rk2 with flags 0) with a hipEvent
- * pair around every launch, on the context's stream; waits for completion.
+/* Eager RK2 run of nbb substeps (as sw_rk2 with flags 0) on the context's
+ * stream, each launch repeated `reps` times (every launch is idempotent) and
+ * timed by start / stop hipEvents attached to its own dispatch
+ * (hipExtLaunchKernelGGL); waits for completion.
  * ms[k] / counts[k] = total milliseconds / launches of kernel class k:
  * 0 edge lines, 1 height stage kernel, 2 height halo fill, 3 velocity stage
  * kernel, 4 velocity halo fill, 5 diagnostics.  A measurement helper with no
  * counterpart in the reference. */
 int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
-               double dtb, double ms[6], int counts[6]);
+               double dtb, int reps, double ms[6], int counts[6]);
 
 /* Global integrals of a state in binary64 with compensated summation over
  * the interior cells, whatever the state precision:

@@ -74,7 +74,7 @@ def lib() -> ctypes.CDLL:
     L.sw_routing_table.restype = ci
     L.sw_info.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.sw_info.restype = ci
-    L.sw_profile.argtypes = [vp, vpp, vpp, ci, ctypes.c_double,
+    L.sw_profile.argtypes = [vp, vpp, vpp, ci, ctypes.c_double, ci,
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ci)]
     L.sw_profile.restype = ci
     L.sw_integrals.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]

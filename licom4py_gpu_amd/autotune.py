@@ -47,7 +47,7 @@ def save_cache(cache: Dict[str, dict]) -> None:
         pass
 
 
-def tune(ctx, core6: Sequence, dt: float, substeps: int = 4, reps: int = 2,
+def tune(ctx, core6: Sequence, dt: float, substeps: int = 2, reps: int = 2,
          candidates: Iterable[int] = CANDIDATES, use_cache: bool = True) -> dict:
     """Pick (rows_a, rows_b) for `ctx` by timing; applies and returns it."""
     import torch
@@ -82,3 +82,38 @@ def tune(ctx, core6: Sequence, dt: float, substeps: int = 4, reps: int = 2,
         cache[key] = result
         save_cache(cache)
     return dict(result, cached=False)
+
+
+def tune_batch(batch, members, candidates: Iterable[int] = CANDIDATES, substeps: int = 4,
+               reps: int = 2) -> dict:
+    """Pick one band height for every member of a batched run by timing the
+    whole batch (short bands help a resolution alone, but in a batch the
+    launches are full anyway and short bands only add warm-up rows).
+    Applies and returns it; the members' own substep counts are restored."""
+    import torch
+
+    saved = [m.nbb for m in members]
+    times = {}
+    try:
+        for m in members:
+            m.nbb = substeps
+        for r in candidates:
+            for m in members:
+                m.ctx.set_tuning(r, r)
+            batch.run(members, diagnostics=False)
+            torch.cuda.synchronize()
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                batch.run(members, diagnostics=False)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            times[r] = best
+    finally:
+        for m, n in zip(members, saved):
+            m.nbb = n
+    r = min(times, key=times.get)
+    for m in members:
+        m.ctx.set_tuning(r, r)
+    return {"rows": r, "table_ms": {str(k): v * 1e3 for k, v in times.items()}}
+

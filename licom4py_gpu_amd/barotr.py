@@ -148,9 +148,10 @@ class SWContext:
         _hip.check(fn(self._h, pin, pout, int(nbb), float(dtb), flags), f"sw_rk{rk}")
         return outs
 
-    def profile(self, core6: Sequence[torch.Tensor], nbb: int, dtb: float):
-        """Eager RK2 run with a hipEvent pair around every launch (on the
-        launch stream); returns {kernel class: (total ms, launches)}."""
+    def profile(self, core6: Sequence[torch.Tensor], nbb: int, dtb: float, reps: int = 1):
+        """Eager RK2 run, every launch (repeated `reps` times) timed by
+        start / stop hipEvents on its own dispatch; returns
+        {kernel class: (total ms, launches)}."""
         ins = [self._check_field(x, STATE_KEYS[k]) for k, x in enumerate(core6)]
         outs = self.empty(13)
         self._bind_stream()
@@ -158,7 +159,9 @@ class SWContext:
         pout, _k2 = _hip.ptr_array([x.data_ptr() for x in outs])
         ms = (ctypes.c_double * 6)()
         cnt = (ctypes.c_int * 6)()
-        _hip.check(_hip.lib().sw_profile(self._h, pin, pout, int(nbb), float(dtb), ms, cnt), "sw_profile")
+        _hip.check(_hip.lib().sw_profile(self._h, pin, pout, int(nbb), float(dtb), int(reps), ms, cnt),
+                   "sw_profile")
+        self.last_profile_out = outs  # the pass computes the same outputs as rk()
         return {name: (ms[k], cnt[k]) for k, name in enumerate(_hip.KERNEL_CLASSES)}
 
     def integrals(self, h: torch.Tensor, u: torch.Tensor, v: torch.Tensor):

@@ -28,6 +28,7 @@
 // -ffp-contract=off, so the binary64 path reproduces the NumPy restatement of
 // the reference exactly.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -1519,10 +1520,11 @@ struct sw_ctx {
   };
   std::vector<GraphEntry> graphs;  // recorded substep-pair sequences
   struct ProfRec {
-    int cls;
+    int cls, reps;
     hipEvent_t a, b;
   };
   std::vector<ProfRec>* prof = nullptr;  // set only by sw_profile
+  int prof_reps = 1;                     // back-to-back launches per timed launch
 };
 
 // Reusable state of sw_batch_run: the device copy of the member tables and
@@ -1559,19 +1561,26 @@ Planes<T> planes_of(const sw_ctx* c) {
 // Kernel classes timed by sw_profile.
 enum { K_EDGE = 0, K_MARCH_A = 1, K_HALO_H = 2, K_MARCH_B = 3, K_HALO_UV = 4, K_DIAG = 5, K_CLASSES = 6 };
 
-int prof_begin(const sw_ctx* c, int cls) {
-  if (!c->prof) return SW_OK;
-  sw_ctx::ProfRec r{cls, nullptr, nullptr};
-  SW_HIP(hipEventCreate(&r.a));
-  SW_HIP(hipEventCreate(&r.b));
-  SW_HIP(hipEventRecord(r.a, c->stream));
-  c->prof->push_back(r);
-  return SW_OK;
-}
-
-int prof_end(const sw_ctx* c) {
-  if (!c->prof) return SW_OK;
-  SW_HIP(hipEventRecord(c->prof->back().b, c->stream));
+// Launch `kernel` on the context's stream.  Under sw_profile the launch is
+// repeated prof_reps times (every launch of the step is idempotent) and each
+// launch carries a start / stop hipEvent on its own dispatch
+// (hipExtLaunchKernelGGL), so the events time the kernel itself with no
+// marker packets between kernels.
+template <typename K, typename A>
+int launch(const sw_ctx* c, int cls, K kernel, dim3 grid, const A& args) {
+  if (!c->prof) {
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, args);
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+  }
+  for (int r = 0; r < c->prof_reps; ++r) {
+    sw_ctx::ProfRec rec{cls, 1, nullptr, nullptr};
+    SW_HIP(hipEventCreate(&rec.a));
+    SW_HIP(hipEventCreate(&rec.b));
+    c->prof->push_back(rec);
+    hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, rec.a, rec.b, 0, args);
+    SW_HIP(hipGetLastError());
+  }
   return SW_OK;
 }
 
@@ -1708,33 +1717,14 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   StageArgs<T> A;
   int rc = stage_args<T>(c, h, hp, u, v, up, vp, h_out, u_out, v_out, dt, beta, last, A);
   if (rc) return rc;
-  if ((rc = prof_begin(c, K_EDGE))) return rc;
-  hipLaunchKernelGGL(k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), dim3(256), 0,
-                     c->stream, A.e);
-  SW_HIP(hipGetLastError());
-  if ((rc = prof_end(c))) return rc;
-  if ((rc = prof_begin(c, K_MARCH_A))) return rc;
-  if (c->gsym)
-    hipLaunchKernelGGL((k_march_a<T, true>), dim3((A.ma.nwaves + 3) / 4), dim3(256), 0, c->stream, A.ma);
-  else
-    hipLaunchKernelGGL((k_march_a<T, false>), dim3((A.ma.nwaves + 3) / 4), dim3(256), 0, c->stream, A.ma);
-  SW_HIP(hipGetLastError());
-  if ((rc = prof_end(c))) return rc;
-  if ((rc = prof_begin(c, K_HALO_H))) return rc;
-  hipLaunchKernelGGL(k_halo_scalar<T>, dim3(blocks_for(c->plan_n)), dim3(256), 0, c->stream, A.hh);
-  SW_HIP(hipGetLastError());
-  if ((rc = prof_end(c))) return rc;
-  if ((rc = prof_begin(c, K_MARCH_B))) return rc;
-  if (c->gsym)
-    hipLaunchKernelGGL((k_march_b<T, true>), dim3((A.mb.nwaves + 3) / 4), dim3(256), 0, c->stream, A.mb);
-  else
-    hipLaunchKernelGGL((k_march_b<T, false>), dim3((A.mb.nwaves + 3) / 4), dim3(256), 0, c->stream, A.mb);
-  SW_HIP(hipGetLastError());
-  if ((rc = prof_end(c))) return rc;
-  if ((rc = prof_begin(c, K_HALO_UV))) return rc;
-  hipLaunchKernelGGL(k_halo_vector<T>, dim3(blocks_for(c->plan_n)), dim3(256), 0, c->stream, A.huv);
-  SW_HIP(hipGetLastError());
-  return prof_end(c);
+  if ((rc = launch(c, K_EDGE, k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), A.e))) return rc;
+  const dim3 ga((A.ma.nwaves + 3) / 4), gb((A.mb.nwaves + 3) / 4), gh(blocks_for(c->plan_n));
+  rc = c->gsym ? launch(c, K_MARCH_A, k_march_a<T, true>, ga, A.ma) : launch(c, K_MARCH_A, k_march_a<T, false>, ga, A.ma);
+  if (rc) return rc;
+  if ((rc = launch(c, K_HALO_H, k_halo_scalar<T>, gh, A.hh))) return rc;
+  rc = c->gsym ? launch(c, K_MARCH_B, k_march_b<T, true>, gb, A.mb) : launch(c, K_MARCH_B, k_march_b<T, false>, gb, A.mb);
+  if (rc) return rc;
+  return launch(c, K_HALO_UV, k_halo_vector<T>, gh, A.huv);
 }
 
 template <typename T>
@@ -1746,11 +1736,7 @@ int run_diag(const sw_ctx* c, const T* h, const T* u, const T* v, void* const ou
   d.evel = edge_buf<T>(c, 0);
   T** dst[10] = {&d.cx, &d.cy, &d.uct, &d.vct, &d.ucx, &d.ucy, &d.vcx, &d.vcy, &d.advx, &d.advy};
   for (int k = 0; k < 10; ++k) *dst[k] = static_cast<T*>(out[3 + k]);
-  int rc = prof_begin(c, K_DIAG);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_diag<T>, dim3(blocks_for(c->cells)), dim3(256), 0, c->stream, d);
-  SW_HIP(hipGetLastError());
-  return prof_end(c);
+  return launch(c, K_DIAG, k_diag<T>, dim3(blocks_for(c->cells)), d);
 }
 
 int check_ptrs(const void* const* p, int n, const char* what) {
@@ -2416,10 +2402,12 @@ int sw_routing_table(int32_t out[6 * 4 * 3]) {
 }
 
 int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb, double dtb,
-               double ms[6], int counts[6]) {
+               int reps, double ms[6], int counts[6]) {
   if (!ctx || !ms || !counts) return fail(SW_E_ARG, "null argument");
+  if (reps < 1 || reps > 1000) return fail(SW_E_ARG, "reps must be in [1, 1000]");
   std::vector<sw_ctx::ProfRec> recs;
   ctx->prof = &recs;
+  ctx->prof_reps = reps;
   int rc = ctx->dtype == SW_F64 ? run_rk<double>(ctx, in, out, nbb, dtb, 0, 2)
                                 : run_rk<float>(ctx, in, out, nbb, dtb, 0, 2);
   ctx->prof = nullptr;
@@ -2432,7 +2420,7 @@ int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nb
     float t = 0.0f;
     if (!rc && e == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
       ms[r.cls] += t;
-      counts[r.cls] += 1;
+      counts[r.cls] += r.reps;
     }
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
