@@ -432,7 +432,9 @@ __device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRa
   }
 }
 
-template <typename T, bool GSYM>
+// SAMEP: hp is h (stage 1 of a substep that starts from its own output), so
+// hp at row i-1 is already in the lane's h window and is not loaded.
+template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void a_prefetch(const MarchArgs<T>& a, const WaveTile& w, int i, int hcol,
                                            APre<T>& p) {
   const int N = a.g.N, nx = a.g.nx;
@@ -453,7 +455,7 @@ __device__ __forceinline__ void a_prefetch(const MarchArgs<T>& a, const WaveTile
   const uint32_t om = static_cast<uint32_t>(ii) * NS + jl;
   p.rda = ldg(P.rda, w.mo + om);
   p.ddx = ldg(P.ddx, w.mo + om);
-  p.hp = ldg(a.hp, w.fo + om);
+  if constexpr (!SAMEP) p.hp = ldg(a.hp, w.fo + om);
   p.dzy = ldg(P.dzy, P.dz_uniform ? 0 : w.fo + om);
   const bool jedge = (j == HALO || j == N - HALO) && ii >= HALO && ii < N - HALO;
   const int je = jedge ? (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
@@ -478,7 +480,7 @@ __device__ __forceinline__ T face_flux2(const T* s, const T* w, bool edge, T upr
   return dz * uc + (uc > T(0) ? he : hw) * uc;
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void march_a_body(const MarchArgs<T>& a, int L, T (*s_v)[kRowBuf],
                                              T (*s_h)[kRowBuf]) {
   WaveTile w;
@@ -506,13 +508,13 @@ __device__ __forceinline__ void march_a_body(const MarchArgs<T>& a, int L, T (*s
     wh[m] = ldg(a.h, o);
   }
   APre<T> pre;
-  a_prefetch<T, GSYM>(a, w, w.r0, hcol, pre);
+  a_prefetch<T, GSYM, SAMEP>(a, w, w.r0, hcol, pre);
 
   // step 0: the i-face at row r0 only
   T Fprev, cdy_prev;
   {
     const APre<T> cur = pre;
-    a_prefetch<T, GSYM>(a, w, w.r0 + 1, hcol, pre);
+    a_prefetch<T, GSYM, SAMEP>(a, w, w.r0 + 1, hcol, pre);
     wu[5] = ct_u(cur.cr);
     wvc[5] = ct_v(cur.cr);
     wh[5] = cur.h;
@@ -531,7 +533,7 @@ __device__ __forceinline__ void march_a_body(const MarchArgs<T>& a, int L, T (*s
 
   for (int i = w.r0 + 1; i <= w.r1; ++i) {
     const APre<T> cur = pre;
-    a_prefetch<T, GSYM>(a, w, i + 1, hcol, pre);
+    a_prefetch<T, GSYM, SAMEP>(a, w, i + 1, hcol, pre);
     wu[5] = ct_u(cur.cr);
     wvc[5] = ct_v(cur.cr);
     wh[5] = cur.h;
@@ -563,7 +565,7 @@ __device__ __forceinline__ void march_a_body(const MarchArgs<T>& a, int L, T (*s
     // divergence (agrid.py:44-53) and height update (barotr_pure.py:120)
     const T D = cur.rda * (((F * cur.cdy - Fprev * cdy_prev) + Gn * ddxn) - G * cur.ddx);
     T* dst = out_lane ? at32(a.h_out, w.fo + static_cast<uint32_t>(ii) * NS + j) : sink;
-    *dst = cur.hp - D * a.dt;
+    *dst = (SAMEP ? wh[2] : cur.hp) - D * a.dt;  // wh[2]: h at row ii
 
     Fprev = F;
     cdy_prev = cur.cdy;
@@ -646,7 +648,8 @@ __device__ __forceinline__ void b_fetch_halo(const MarchArgs<T>& a, const WaveTi
   p.h = ldg(a.h, o);
   p.hn = ldg(a.hn, o);
 }
-template <typename T>
+// SAMEP: up, vp are u, v, whose row ii is already in the lane's windows.
+template <typename T, bool SAMEP>
 __device__ __forceinline__ void b_fetch_cell(const MarchArgs<T>& a, const WaveTile& w, int ii, BCell<T>& p) {
   const int N = a.g.N, nx = a.g.nx;
   const Planes<T>& P = a.P;
@@ -656,8 +659,10 @@ __device__ __forceinline__ void b_fetch_cell(const MarchArgs<T>& a, const WaveTi
   p.rdx = ldg(P.rdx, w.mo + om);
   p.rdy = ldg(P.rdy, w.mo + om);
   p.af = ldg(P.af, w.fo + om);
-  p.up = ldg(a.up, w.fo + om);
-  p.vp = ldg(a.vp, w.fo + om);
+  if constexpr (!SAMEP) {
+    p.up = ldg(a.up, w.fo + om);
+    p.vp = ldg(a.vp, w.fo + om);
+  }
   p.dzy = ldg(P.dzy, P.dz_uniform ? 0 : w.fo + om);
   const bool jedge = (w.j == HALO || w.j == N - HALO) && ii >= HALO && ii < N - HALO;
   p.evj = ldg(a.evel, jedge ? (w.t * 4 + (w.j == HALO ? DW : DE)) * nx + (ii - HALO) : 0);
@@ -682,7 +687,7 @@ __device__ __forceinline__ void iface_b(const T* wu, const T* wv, const T* wh, b
   Cx = T(0.5) * cx * (ue - uw);
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s_vct)[kRowBuf],
                                              T (*s_u)[kRowBuf], T (*s_h)[kRowBuf], T (*s_n)[kRowBuf]) {
   WaveTile w;
@@ -742,7 +747,7 @@ __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s
     iface_b(wu, wv, wh, (i == HALO) || (i == N - HALO), f.evi, f.dzx, ucx_prev, vcx_prev, Cx_prev);
     cdy_prev = f.cdy;
     b_fetch_halo<T, GSYM>(a, w, w.r0, hcol, phalo);
-    b_fetch_cell(a, w, w.r0, pcell);
+    b_fetch_cell<T, SAMEP>(a, w, w.r0, pcell);
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
       wu[m] = wu[m + 1];
@@ -833,9 +838,10 @@ __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s
     const uint32_t oc = w.fo + static_cast<uint32_t>(ii) * NS + j;
     T* du = out_lane ? at32(a.u_out, oc) : sink;
     T* dv = out_lane ? at32(a.v_out, oc) : sink;
-    *du = cl.up + (((pgfu + advx) + cl.af * vct_c) + visx) * a.dt;
-    *dv = cl.vp + (((pgfv + advy) - cl.af * uct_c) + visy) * a.dt;
-    b_fetch_cell(a, w, i, pcell);
+    // wuu[0], wv[2]: u, v at row ii
+    *du = (SAMEP ? wuu[0] : cl.up) + (((pgfu + advx) + cl.af * vct_c) + visx) * a.dt;
+    *dv = (SAMEP ? wv[2] : cl.vp) + (((pgfv + advy) - cl.af * uct_c) + visy) * a.dt;
+    b_fetch_cell<T, SAMEP>(a, w, i, pcell);
 
     ucx_prev = ucx;
     vcx_prev = vcx;
@@ -857,20 +863,20 @@ __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s
   }
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, bool SAMEP>
 __global__ __launch_bounds__(256) void k_march_a(MarchArgs<T> a) {
   __shared__ T s_v[4][kRowBuf];
   __shared__ T s_h[4][kRowBuf];
-  march_a_body<T, GSYM>(a, xcd_block(), s_v, s_h);
+  march_a_body<T, GSYM, SAMEP>(a, xcd_block(), s_v, s_h);
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, bool SAMEP>
 __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
   __shared__ T s_vct[4][kRowBuf];
   __shared__ T s_u[4][kRowBuf];
   __shared__ T s_h[4][kRowBuf];
   __shared__ T s_n[4][kRowBuf];
-  march_b_body<T, GSYM>(a, xcd_block(), s_vct, s_u, s_h, s_n);
+  march_b_body<T, GSYM, SAMEP>(a, xcd_block(), s_vct, s_u, s_h, s_n);
 }
 
 // ---------------------------------------------------------------------------
@@ -1116,17 +1122,17 @@ __global__ __launch_bounds__(256) void k_edge_batch(BatchLaunch<T> b) {
   edge_body(a, (blk - b.first[m]) * 256 + static_cast<int>(threadIdx.x));
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, bool SAMEP>
 __global__ __launch_bounds__(256) void k_march_a_batch(BatchLaunch<T> b) {
   __shared__ T s_v[4][kRowBuf];
   __shared__ T s_h[4][kRowBuf];
   const int L = xcd_block();
   const int m = b.blk2m[L];
   const MarchArgs<T> a = batch_entry(b, m).ma;
-  march_a_body<T, GSYM>(a, L - b.first[m], s_v, s_h);
+  march_a_body<T, GSYM, SAMEP>(a, L - b.first[m], s_v, s_h);
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, bool SAMEP>
 __global__ __launch_bounds__(256) void k_march_b_batch(BatchLaunch<T> b) {
   __shared__ T s_vct[4][kRowBuf];
   __shared__ T s_u[4][kRowBuf];
@@ -1135,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_march_b_batch(BatchLaunch<T> b) {
   const int L = xcd_block();
   const int m = b.blk2m[L];
   const MarchArgs<T> a = batch_entry(b, m).mb;
-  march_b_body<T, GSYM>(a, L - b.first[m], s_vct, s_u, s_h, s_n);
+  march_b_body<T, GSYM, SAMEP>(a, L - b.first[m], s_vct, s_u, s_h, s_n);
 }
 
 template <typename T>
@@ -1719,10 +1725,19 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   if (rc) return rc;
   if ((rc = launch(c, K_EDGE, k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), A.e))) return rc;
   const dim3 ga((A.ma.nwaves + 3) / 4), gb((A.mb.nwaves + 3) / 4), gh(blocks_for(c->plan_n));
-  rc = c->gsym ? launch(c, K_MARCH_A, k_march_a<T, true>, ga, A.ma) : launch(c, K_MARCH_A, k_march_a<T, false>, ga, A.ma);
+  const bool sa = A.ma.hp == A.ma.h, sb = A.mb.up == A.mb.u && A.mb.vp == A.mb.v;
+  auto la = [&](auto k) { return launch(c, K_MARCH_A, k, ga, A.ma); };
+  auto lb = [&](auto k) { return launch(c, K_MARCH_B, k, gb, A.mb); };
+  if (c->gsym)
+    rc = sa ? la(k_march_a<T, true, true>) : la(k_march_a<T, true, false>);
+  else
+    rc = sa ? la(k_march_a<T, false, true>) : la(k_march_a<T, false, false>);
   if (rc) return rc;
   if ((rc = launch(c, K_HALO_H, k_halo_scalar<T>, gh, A.hh))) return rc;
-  rc = c->gsym ? launch(c, K_MARCH_B, k_march_b<T, true>, gb, A.mb) : launch(c, K_MARCH_B, k_march_b<T, false>, gb, A.mb);
+  if (c->gsym)
+    rc = sb ? lb(k_march_b<T, true, true>) : lb(k_march_b<T, true, false>);
+  else
+    rc = sb ? lb(k_march_b<T, false, true>) : lb(k_march_b<T, false, false>);
   if (rc) return rc;
   return launch(c, K_HALO_UV, k_halo_vector<T>, gh, A.huv);
 }
@@ -2075,6 +2090,11 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return mem[x].nbb > mem[y].nbb; });
   int K = 0;
   while (K < count && mem[order[K]].nbb > 0) ++K;
+  bool in_alias = true;  // every stepping member's input has h0p = h0, ubp = ub, vbp = vb
+  for (int q = 0; q < K; ++q) {
+    const sw_member& m = mem[order[q]];
+    in_alias = in_alias && m.in[1] == m.in[0] && m.in[4] == m.in[2] && m.in[5] == m.in[3];
+  }
   for (int i = K; i < count; ++i) {  // nbb = 0: the reference's empty loop returns the input
     const sw_member& m = mem[order[i]];
     const size_t bytes = m.ctx->cells * sizeof(T);
@@ -2205,20 +2225,29 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
         l.parity = k & 1;
       }
       hipLaunchKernelGGL(k_edge_batch<T>, dim3(first[BK_EDGE][active]), dim3(256), 0, stream, L[BK_EDGE]);
-      if (gsym)
-        hipLaunchKernelGGL((k_march_a_batch<T, true>), dim3(first[BK_MARCH_A][active]), dim3(256), 0, stream,
-                           L[BK_MARCH_A]);
+      // stage 1 of a substep reads its *p fields at the addresses of h, u, v
+      // for every member, except on the first substep unless every member's
+      // input aliases them
+      const bool samep = st == 0 && (k > 0 || in_alias);
+      const dim3 ga(first[BK_MARCH_A][active]), gb(first[BK_MARCH_B][active]);
+      if (gsym && samep)
+        hipLaunchKernelGGL((k_march_a_batch<T, true, true>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
+      else if (gsym)
+        hipLaunchKernelGGL((k_march_a_batch<T, true, false>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
+      else if (samep)
+        hipLaunchKernelGGL((k_march_a_batch<T, false, true>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
       else
-        hipLaunchKernelGGL((k_march_a_batch<T, false>), dim3(first[BK_MARCH_A][active]), dim3(256), 0, stream,
-                           L[BK_MARCH_A]);
+        hipLaunchKernelGGL((k_march_a_batch<T, false, false>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
       hipLaunchKernelGGL(k_halo_scalar_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, stream,
                          L[BK_HALO]);
-      if (gsym)
-        hipLaunchKernelGGL((k_march_b_batch<T, true>), dim3(first[BK_MARCH_B][active]), dim3(256), 0, stream,
-                           L[BK_MARCH_B]);
+      if (gsym && samep)
+        hipLaunchKernelGGL((k_march_b_batch<T, true, true>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
+      else if (gsym)
+        hipLaunchKernelGGL((k_march_b_batch<T, true, false>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
+      else if (samep)
+        hipLaunchKernelGGL((k_march_b_batch<T, false, true>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
       else
-        hipLaunchKernelGGL((k_march_b_batch<T, false>), dim3(first[BK_MARCH_B][active]), dim3(256), 0, stream,
-                           L[BK_MARCH_B]);
+        hipLaunchKernelGGL((k_march_b_batch<T, false, false>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
       hipLaunchKernelGGL(k_halo_vector_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, stream,
                          L[BK_HALO]);
       SW_HIP(hipGetLastError());
