@@ -8,25 +8,25 @@
 // face's memory.
 //
 // One RK stage (momentum/barotr_pure.py:100-168) runs as five launches:
-//   k_edge     C-grid normal velocity on every face edge, averaged with the
-//              neighbour's (remap.py:107 -> communication.py:485-564); in the
-//              last stage also the averaged mass flux (barotr_pure.py:116-117).
-//              Each edge line is computed by both faces from the same two
-//              operands, so both sides agree bit for bit.
-//   k_stage_a  new height on the interior: celerity, C-grid velocity, LMARS
-//              correction, upwind flux, divergence (barotr_pure.py:111-120).
-//   k_halo_h   duo-grid halo fill of the new height (cube.py:130-134).
-//   k_stage_b  new covariant velocity on the interior: D-grid upwind,
-//              vorticity/kinetic-energy advection, LMARS dissipation, pressure
-//              gradient, Coriolis (barotr_pure.py:133-150).
-//   k_halo_uv  vector halo fill (cube.py:137-153).
+//   k_edge         C-grid normal velocity on the 12 cube edges, averaged over
+//                  the two faces that meet there (remap.py:107 ->
+//                  communication.py:485-564); in the last stage also the
+//                  averaged mass flux (barotr_pure.py:116-117).
+//   k_march_a      new height on the interior: celerity, C-grid velocity, LMARS
+//                  correction, upwind flux, divergence (barotr_pure.py:111-120).
+//   k_halo_scalar  duo-grid halo fill of the new height (cube.py:130-134).
+//   k_march_b      new covariant velocity on the interior: D-grid upwind,
+//                  vorticity/kinetic-energy advection, LMARS dissipation,
+//                  pressure gradient, Coriolis (barotr_pure.py:133-150).
+//   k_halo_vector  vector halo fill (cube.py:137-153).
 // The ten diagnostic outputs of the last stage are written once per call by
-// k_diag over the full padded arrays, with the reference's values on the
-// halo ring.  Stage kernels stage their stencil footprints (height, velocity
-// components, contravariant velocity) in LDS tiles; every multiply and add is
-// evaluated in the reference's order and the library is compiled with
-// -ffp-contract=off, so the binary64 path reproduces the NumPy restatement of
-// the reference exactly.
+// k_diag1..3 over the full padded arrays, with the reference's values on the
+// halo ring.  The stage kernels march wavefronts down columns with the
+// stencil windows in registers and one LDS row buffer per wave; every
+// multiply and add is evaluated in the reference's order and the library is
+// compiled with -ffp-contract=off, so the binary64 path reproduces the NumPy
+// restatement of the reference exactly.  *_batch variants advance several
+// contexts per launch.
 
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
@@ -1243,70 +1243,14 @@ struct DiagCell {
       s = c6(Hh(i, j - 3), Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1), Hh(i, j + 2));
     return celerity(s, dzy(P, g.at(t, i, j)));
   }
-  // averaged, uncorrected C-grid u (valid for 3 <= i < N-2)
-  __device__ T ucx_pre(int i, int j) const {
-    const int N = g.N;
-    if ((i == 3 || i == N - 3) && j >= 3 && j < N - 3)
-      return evel[(t * 4 + (i == 3 ? DS : DN)) * g.nx + (j - 3)];
-    return T(0.5) * (e5(U(i - 3, j), U(i - 2, j), U(i - 1, j), U(i, j), U(i + 1, j)) +
-                     w5(U(i - 2, j), U(i - 1, j), U(i, j), U(i + 1, j), U(i + 2, j)));
-  }
-  __device__ T vcy_pre(int i, int j) const {
-    const int N = g.N;
-    if ((j == 3 || j == N - 3) && i >= 3 && i < N - 3)
-      return evel[(t * 4 + (j == 3 ? DW : DE)) * g.nx + (i - 3)];
-    return T(0.5) * (e5(V(i, j - 3), V(i, j - 2), V(i, j - 1), V(i, j), V(i, j + 1)) +
-                     w5(V(i, j - 2), V(i, j - 1), V(i, j), V(i, j + 1), V(i, j + 2)));
-  }
-  __device__ T ucx(int i, int j) const {
-    const int N = g.N;
-    if (i < 3 || i >= N - 2) return T(0);
-    const T he = e5(Hh(i - 3, j), Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j));
-    const T hw = w5(Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j), Hh(i + 2, j));
-    return ucx_pre(i, j) + T(4.9) / cx(i, j) * (he - hw);
-  }
-  __device__ T vcy(int i, int j) const {
-    const int N = g.N;
-    if (j < 3 || j >= N - 2) return T(0);
-    const T hn = e5(Hh(i, j - 3), Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1));
-    const T hs = w5(Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1), Hh(i, j + 2));
-    return vcy_pre(i, j) + T(4.9) / cy(i, j) * (hn - hs);
-  }
-  __device__ T vcx(int i, int j) const {
-    const int N = g.N;
-    if (i < 3 || i >= N - 2) return T(0);
-    const size_t b = g.at(t, 0, j);
-    auto Vv = [&](int ii) { return v[b + static_cast<size_t>(ii) * N]; };
-    return ucx_pre(i, j) > T(0) ? e5(Vv(i - 3), Vv(i - 2), Vv(i - 1), Vv(i), Vv(i + 1))
-                                : w5(Vv(i - 2), Vv(i - 1), Vv(i), Vv(i + 1), Vv(i + 2));
-  }
-  __device__ T ucy(int i, int j) const {
-    const int N = g.N;
-    if (j < 3 || j >= N - 2) return T(0);
-    const size_t b = g.at(t, i, 0);
-    auto Uu = [&](int jj) { return u[b + jj]; };
-    return vcy_pre(i, j) > T(0) ? e5(Uu(j - 3), Uu(j - 2), Uu(j - 1), Uu(j), Uu(j + 1))
-                                : w5(Uu(j - 2), Uu(j - 1), Uu(j), Uu(j + 1), Uu(j + 2));
-  }
-  __device__ T zeta(int i, int j) const {
-    const int N = g.N;
-    if (i >= N - 1 || j >= N - 1) return T(0);
-    const size_t o = g.at(t, i, j);
-    return P.rda[o] * (((ucy(i, j) * P.ddx[o] + vcx(i + 1, j) * P.cdy[o + N]) - ucy(i, j + 1) * P.ddx[o + 1]) -
-                       vcx(i, j) * P.cdy[o]);
-  }
-  __device__ T ku(int i, int j) const {
-    const T a = ucx(i, j), b = vcx(i, j);
-    return T(0.5) * (a * a + b * b);
-  }
-  __device__ T kv(int i, int j) const {
-    const T a = vcy(i, j), b = ucy(i, j);
-    return T(0.5) * (a * a + b * b);
-  }
 };
 
+// The diagnostics in three passes over the padded arrays.  Each pass stores
+// outputs that the next reads instead of recomputing them at every
+// neighbour; every value is the reference's expression in its order.
+// Pass 1: celerity and contravariant velocity.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag(DiagArgs<T> a) {
+__global__ __launch_bounds__(256) void k_diag1(DiagArgs<T> a) {
   const Geo g = a.g;
   const int N = g.N;
   const size_t NN = static_cast<size_t>(N) * N;
@@ -1318,18 +1262,85 @@ __global__ __launch_bounds__(256) void k_diag(DiagArgs<T> a) {
   const DiagCell<T> C{g, a.P, a.h, a.u, a.v, a.evel, t};
   a.cx[q] = C.cx(i, j);
   a.cy[q] = C.cy(i, j);
-  const T uct = C.U(i, j), vct = C.V(i, j);
-  a.uct[q] = uct;
-  a.vct[q] = vct;
-  a.ucx[q] = C.ucx(i, j);
-  a.vcy[q] = C.vcy(i, j);
-  a.vcx[q] = C.vcx(i, j);
-  a.ucy[q] = C.ucy(i, j);
+  a.uct[q] = C.U(i, j);
+  a.vct[q] = C.V(i, j);
+}
+
+// Pass 2: corrected C-grid velocities (remap.py:42-55, barotr_pure.py:29-34)
+// and upwind D-grid velocities (remap.py:74-95) from the stored uct, vct, cx, cy.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag2(DiagArgs<T> a) {
+  const Geo g = a.g;
+  const int N = g.N;
+  const size_t NN = static_cast<size_t>(N) * N;
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= 6 * NN) return;
+  const int t = static_cast<int>(q / NN);
+  const int i = static_cast<int>((q % NN) / N);
+  const int j = static_cast<int>(q % N);
+  const size_t f0 = static_cast<size_t>(t) * NN;
+  auto U = [&](int ii, int jj) { return a.uct[f0 + static_cast<size_t>(ii) * N + jj]; };
+  auto V = [&](int ii, int jj) { return a.vct[f0 + static_cast<size_t>(ii) * N + jj]; };
+  auto Hh = [&](int ii, int jj) { return a.h[f0 + static_cast<size_t>(ii) * N + jj]; };
+  T ucx = T(0), vcx = T(0), vcy = T(0), ucy = T(0);
+  if (i >= 3 && i < N - 2) {
+    T pre;
+    if ((i == 3 || i == N - 3) && j >= 3 && j < N - 3)
+      pre = a.evel[(t * 4 + (i == 3 ? DS : DN)) * g.nx + (j - 3)];
+    else
+      pre = T(0.5) * (e5(U(i - 3, j), U(i - 2, j), U(i - 1, j), U(i, j), U(i + 1, j)) +
+                      w5(U(i - 2, j), U(i - 1, j), U(i, j), U(i + 1, j), U(i + 2, j)));
+    const T he = e5(Hh(i - 3, j), Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j));
+    const T hw = w5(Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j), Hh(i + 2, j));
+    ucx = pre + T(4.9) / a.cx[q] * (he - hw);
+    const size_t b = f0 + j;
+    auto Vv = [&](int ii) { return a.v[b + static_cast<size_t>(ii) * N]; };
+    vcx = pre > T(0) ? e5(Vv(i - 3), Vv(i - 2), Vv(i - 1), Vv(i), Vv(i + 1))
+                     : w5(Vv(i - 2), Vv(i - 1), Vv(i), Vv(i + 1), Vv(i + 2));
+  }
+  if (j >= 3 && j < N - 2) {
+    T pre;
+    if ((j == 3 || j == N - 3) && i >= 3 && i < N - 3)
+      pre = a.evel[(t * 4 + (j == 3 ? DW : DE)) * g.nx + (i - 3)];
+    else
+      pre = T(0.5) * (e5(V(i, j - 3), V(i, j - 2), V(i, j - 1), V(i, j), V(i, j + 1)) +
+                      w5(V(i, j - 2), V(i, j - 1), V(i, j), V(i, j + 1), V(i, j + 2)));
+    const T hn = e5(Hh(i, j - 3), Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1));
+    const T hs = w5(Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1), Hh(i, j + 2));
+    vcy = pre + T(4.9) / a.cy[q] * (hn - hs);
+    const size_t b = f0 + static_cast<size_t>(i) * N;
+    auto Uu = [&](int jj) { return a.u[b + jj]; };
+    ucy = pre > T(0) ? e5(Uu(j - 3), Uu(j - 2), Uu(j - 1), Uu(j), Uu(j + 1))
+                     : w5(Uu(j - 2), Uu(j - 1), Uu(j), Uu(j + 1), Uu(j + 2));
+  }
+  a.ucx[q] = ucx;
+  a.vcx[q] = vcx;
+  a.vcy[q] = vcy;
+  a.ucy[q] = ucy;
+}
+
+// Pass 3: vorticity and kinetic-energy advection (barotr_pure.py:78-97,
+// agrid.py:32-41) from the stored velocities.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag3(DiagArgs<T> a) {
+  const Geo g = a.g;
+  const int N = g.N;
+  const size_t NN = static_cast<size_t>(N) * N;
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= 6 * NN) return;
+  const int i = static_cast<int>((q % NN) / N);
+  const int j = static_cast<int>(q % N);
+  const Planes<T>& P = a.P;
+  auto ku = [&](size_t o) { return T(0.5) * (a.ucx[o] * a.ucx[o] + a.vcx[o] * a.vcx[o]); };
+  auto kv = [&](size_t o) { return T(0.5) * (a.vcy[o] * a.vcy[o] + a.ucy[o] * a.ucy[o]); };
   T ax = T(0), ay = T(0);
   if (i < N - 1 || j < N - 1) {
-    const T z = C.zeta(i, j);
-    if (i < N - 1) ax = z * vct - (C.ku(i + 1, j) - C.ku(i, j)) * a.P.rdx[q];
-    if (j < N - 1) ay = -z * uct - (C.kv(i, j + 1) - C.kv(i, j)) * a.P.rdy[q];
+    T z = T(0);
+    if (i < N - 1 && j < N - 1)
+      z = P.rda[q] * (((a.ucy[q] * P.ddx[q] + a.vcx[q + N] * P.cdy[q + N]) - a.ucy[q + 1] * P.ddx[q + 1]) -
+                      a.vcx[q] * P.cdy[q]);
+    if (i < N - 1) ax = z * a.vct[q] - (ku(q + N) - ku(q)) * P.rdx[q];
+    if (j < N - 1) ay = -z * a.uct[q] - (kv(q + 1) - kv(q)) * P.rdy[q];
   }
   a.advx[q] = ax;
   a.advy[q] = ay;
@@ -1751,7 +1762,10 @@ int run_diag(const sw_ctx* c, const T* h, const T* u, const T* v, void* const ou
   d.evel = edge_buf<T>(c, 0);
   T** dst[10] = {&d.cx, &d.cy, &d.uct, &d.vct, &d.ucx, &d.ucy, &d.vcx, &d.vcy, &d.advx, &d.advy};
   for (int k = 0; k < 10; ++k) *dst[k] = static_cast<T*>(out[3 + k]);
-  return launch(c, K_DIAG, k_diag<T>, dim3(blocks_for(c->cells)), d);
+  int rc = launch(c, K_DIAG, k_diag1<T>, dim3(blocks_for(c->cells)), d);
+  if (!rc) rc = launch(c, K_DIAG, k_diag2<T>, dim3(blocks_for(c->cells)), d);
+  if (!rc) rc = launch(c, K_DIAG, k_diag3<T>, dim3(blocks_for(c->cells)), d);
+  return rc;
 }
 
 int check_ptrs(const void* const* p, int n, const char* what) {
