@@ -162,7 +162,8 @@ void sw_batch_destroy(sw_batch* b);
  * their own nbb RK substeps (rk = 2 or 3), every kernel class launched once
  * per stage for all members still stepping, on `stream`.  Each member's
  * outputs equal those of sw_rk2 / sw_rk3 on its context alone (flags:
- * SW_SKIP_DIAG).  Generalises the reference's one-field loop
+ * SW_SKIP_DIAG; SW_GRAPH replays a recorded graph of substep pairs when all
+ * members have the same nbb >= 3).  Generalises the reference's one-field loop
  * (momentum/barotr.py:41-87) to many fields per launch.  A batch object may be
  * reused; its tables are replaced at every run. */
 int sw_batch_run(sw_batch* b, const sw_member* members, int count, int rk, int flags, void* stream);

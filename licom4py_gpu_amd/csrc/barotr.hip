@@ -1554,6 +1554,12 @@ struct sw_batch {
   hipEvent_t staged = nullptr;   // recorded after the staging copy of the last run
   hipStream_t stream = nullptr;  // stream of the last run
   bool used = false;
+  hipStream_t capture = nullptr;  // private stream used only to record graphs
+  struct Graph {
+    std::vector<long long> sig;  // table layout, member count, grids
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;     // recorded substep pairs
 };
 
 namespace {
@@ -2229,42 +2235,89 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     L[k].blk2m = reinterpret_cast<const int*>(d + o_blk[k]);
     L[k].count = K;
   }
-  int active = K;
-  for (int k = 0; k < nbb[0]; ++k) {
-    while (active > 0 && nbb[active - 1] <= k) --active;
+  // one substep k of the members [0, active) on stream s
+  auto substep = [&](int k, int active, hipStream_t s) -> int {
     for (int st = 0; st < rk; ++st) {
       for (auto& l : L) {
         l.stage = st;
         l.first_sub = (k == 0);
         l.parity = k & 1;
       }
-      hipLaunchKernelGGL(k_edge_batch<T>, dim3(first[BK_EDGE][active]), dim3(256), 0, stream, L[BK_EDGE]);
+      hipLaunchKernelGGL(k_edge_batch<T>, dim3(first[BK_EDGE][active]), dim3(256), 0, s, L[BK_EDGE]);
       // stage 1 of a substep reads its *p fields at the addresses of h, u, v
       // for every member, except on the first substep unless every member's
       // input aliases them
       const bool samep = st == 0 && (k > 0 || in_alias);
       const dim3 ga(first[BK_MARCH_A][active]), gb(first[BK_MARCH_B][active]);
       if (gsym && samep)
-        hipLaunchKernelGGL((k_march_a_batch<T, true, true>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
+        hipLaunchKernelGGL((k_march_a_batch<T, true, true>), ga, dim3(256), 0, s, L[BK_MARCH_A]);
       else if (gsym)
-        hipLaunchKernelGGL((k_march_a_batch<T, true, false>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
+        hipLaunchKernelGGL((k_march_a_batch<T, true, false>), ga, dim3(256), 0, s, L[BK_MARCH_A]);
       else if (samep)
-        hipLaunchKernelGGL((k_march_a_batch<T, false, true>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
+        hipLaunchKernelGGL((k_march_a_batch<T, false, true>), ga, dim3(256), 0, s, L[BK_MARCH_A]);
       else
-        hipLaunchKernelGGL((k_march_a_batch<T, false, false>), ga, dim3(256), 0, stream, L[BK_MARCH_A]);
-      hipLaunchKernelGGL(k_halo_scalar_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, stream,
-                         L[BK_HALO]);
+        hipLaunchKernelGGL((k_march_a_batch<T, false, false>), ga, dim3(256), 0, s, L[BK_MARCH_A]);
+      hipLaunchKernelGGL(k_halo_scalar_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, s, L[BK_HALO]);
       if (gsym && samep)
-        hipLaunchKernelGGL((k_march_b_batch<T, true, true>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
+        hipLaunchKernelGGL((k_march_b_batch<T, true, true>), gb, dim3(256), 0, s, L[BK_MARCH_B]);
       else if (gsym)
-        hipLaunchKernelGGL((k_march_b_batch<T, true, false>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
+        hipLaunchKernelGGL((k_march_b_batch<T, true, false>), gb, dim3(256), 0, s, L[BK_MARCH_B]);
       else if (samep)
-        hipLaunchKernelGGL((k_march_b_batch<T, false, true>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
+        hipLaunchKernelGGL((k_march_b_batch<T, false, true>), gb, dim3(256), 0, s, L[BK_MARCH_B]);
       else
-        hipLaunchKernelGGL((k_march_b_batch<T, false, false>), gb, dim3(256), 0, stream, L[BK_MARCH_B]);
-      hipLaunchKernelGGL(k_halo_vector_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, stream,
-                         L[BK_HALO]);
+        hipLaunchKernelGGL((k_march_b_batch<T, false, false>), gb, dim3(256), 0, s, L[BK_MARCH_B]);
+      hipLaunchKernelGGL(k_halo_vector_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, s, L[BK_HALO]);
       SW_HIP(hipGetLastError());
+    }
+    return SW_OK;
+  };
+  const int nmax = nbb[0];
+  if ((flags & SW_GRAPH) && nbb[K - 1] == nmax && nmax >= 3) {
+    // every member steps nmax substeps: substep 0 eagerly, then recorded
+    // (odd, even) substep pairs; the graph reads the tables from b->dev, so it
+    // stays valid for any members with the same table layout and grids
+    int rc = substep(0, K, stream);
+    if (rc) return rc;
+    std::vector<long long> sig = {reinterpret_cast<long long>(b->dev), static_cast<long long>(o_tab),
+                                  static_cast<long long>(o_nbb), K, rk, gsym ? 1 : 0,
+                                  static_cast<long long>(sizeof(T))};
+    for (int k = 0; k < BK_CLASSES; ++k) {
+      sig.push_back(static_cast<long long>(o_first[k]));
+      sig.push_back(static_cast<long long>(o_blk[k]));
+      sig.push_back(first[k][K]);
+    }
+    hipGraphExec_t exec = nullptr;
+    for (const auto& g : b->graphs)
+      if (g.sig == sig) exec = g.exec;
+    if (!exec) {
+      if (!b->capture) SW_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+      SW_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+      rc = substep(1, K, b->capture);
+      if (!rc) rc = substep(2, K, b->capture);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->capture, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(SW_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return fail(SW_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      if (b->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(b->graphs.front().exec);
+        b->graphs.erase(b->graphs.begin());
+      }
+      b->graphs.push_back({sig, exec});
+    }
+    for (int p = 0; p < (nmax - 1) / 2; ++p) SW_HIP(hipGraphLaunch(exec, stream));
+    if ((nmax - 1) % 2 && (rc = substep(nmax - 1, K, stream))) return rc;
+  } else {
+    int active = K;
+    for (int k = 0; k < nmax; ++k) {
+      while (active > 0 && nbb[active - 1] <= k) --active;
+      const int rc = substep(k, active, stream);
+      if (rc) return rc;
     }
   }
   // ---- diagnostics of each member from the inputs of its last stage
@@ -2514,6 +2567,8 @@ void sw_batch_destroy(sw_batch* b) {
     (void)hipEventSynchronize(b->staged);
     (void)hipEventDestroy(b->staged);
   }
+  for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+  if (b->capture) (void)hipStreamDestroy(b->capture);
   if (b->dev) (void)hipFree(b->dev);
   if (b->host) (void)hipHostFree(b->host);
   delete b;

@@ -58,10 +58,12 @@ class Batch:
             lib.sw_batch_destroy(h)
             self._h = None
 
-    def run(self, members: Sequence[Member], rk: int = 2, diagnostics: bool = True):
+    def run(self, members: Sequence[Member], rk: int = 2, diagnostics: bool = True, graph: bool = False):
         """Advance every member by its own nbb substeps; fills and returns each
         member's outputs (h0, ub, vb, then the 10 diagnostics unless
-        ``diagnostics`` is false, when 3 output tensors suffice)."""
+        ``diagnostics`` is false, when 3 output tensors suffice).  With
+        ``graph`` and one substep count for all members, the substeps after
+        the first replay a recorded hipGraph of substep pairs."""
         if not members:
             return []
         arr = (_hip.SWMember * len(members))()
@@ -82,7 +84,7 @@ class Batch:
             arr[k].nbb = int(m.nbb)
             arr[k].dtb = float(m.dtb)
         stream = torch.cuda.current_stream(dev).cuda_stream
-        flags = 0 if diagnostics else _hip.SW_SKIP_DIAG
+        flags = (0 if diagnostics else _hip.SW_SKIP_DIAG) | (_hip.SW_GRAPH if graph else 0)
         _hip.check(_hip.lib().sw_batch_run(self._h, arr, len(members), int(rk), flags, ctypes.c_void_p(stream)),
                    "sw_batch_run")
         return [m.out for m in members]
@@ -178,12 +180,12 @@ def main(argv=None) -> int:
     if not a.no_autotune:
         from . import autotune
         batch_tuning = autotune.tune_batch(batch, members)
-    batch.run(members, diagnostics=False)  # warm-up
+    batch.run(members, diagnostics=False, graph=True)  # warm-up, records the graph
     torch.cuda.synchronize()
     tb = []
     for _ in range(a.reps):
         t0 = time.perf_counter()
-        batch.run(members, diagnostics=False)
+        batch.run(members, diagnostics=False, graph=True)
         torch.cuda.synchronize()
         tb.append(time.perf_counter() - t0)
     got = [[x.clone() for x in m.out[:3]] for m in members]

@@ -79,6 +79,20 @@ def test_batch_fp32_matches_alone():
     _check_alone(torch, members)
 
 
+@pytest.mark.parametrize("nbb", [3, 8])
+def test_batch_graph_replay_matches_alone(nbb):
+    """Equal substep counts: substep pairs replay a recorded graph; a second
+    run with other members of the same layout reuses it."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    b = ensemble.Batch()
+    spec = [(8, nbb, 40.0), (13, nbb, 25.0), (16, nbb, 31.0), (16, nbb, 30.0)]
+    for seed in (1, 2):
+        members = _members(torch, torch.float64, spec, seed=seed)
+        b.run(members, graph=True)
+        _check_alone(torch, members)
+
+
 def test_batch_object_reused_with_other_members():
     torch = _torch()
     from licom4py_gpu_amd import ensemble
