@@ -73,6 +73,54 @@ def request_bytes(nx: int, dtype: torch.dtype) -> int:
     return (SCRATCH_FIELDS + STATE_FIELDS) * field_bytes(nx, dtype)
 
 
+class HBMLedger:
+    """Device bytes held under the budget, per kind (nx, dtype) of request:
+    admitted requests hold a workspace and two state buffers; a retired
+    request's workspace stays idle for the next request of its kind until
+    room is needed; a kind's consts stay resident once loaded."""
+
+    def __init__(self, budget_bytes: float):
+        self.budget = float(budget_bytes)
+        self.used = 0.0
+        self.peak = 0.0
+        self.resident = set()
+        self.idle: Dict[tuple, int] = collections.defaultdict(int)
+
+    def need(self, key) -> float:
+        """Bytes admitting a request of `key` adds now."""
+        if self.idle[key]:
+            return STATE_FIELDS * field_bytes(*key)
+        extra = 0 if key in self.resident else CONST_PLANES * field_bytes(*key)
+        return request_bytes(*key) + extra
+
+    def fits(self, key) -> bool:
+        return self.used + self.need(key) <= self.budget
+
+    def admit(self, key) -> bool:
+        """Account an admission; True if it reuses an idle workspace."""
+        self.used += self.need(key)
+        self.peak = max(self.peak, self.used)
+        self.resident.add(key)
+        if self.idle[key]:
+            self.idle[key] -= 1
+            return True
+        return False
+
+    def retire(self, key) -> None:
+        self.used -= STATE_FIELDS * field_bytes(*key)
+        self.idle[key] += 1
+
+    def drop_idle(self):
+        """Release the largest idle workspace; returns its kind, or None."""
+        keys = [k for k, n in self.idle.items() if n]
+        if not keys:
+            return None
+        k = max(keys, key=lambda k: field_bytes(*k))
+        self.idle[k] -= 1
+        self.used -= SCRATCH_FIELDS * field_bytes(*k)
+        return k
+
+
 @dataclass
 class _Active:
     req: Request
@@ -92,9 +140,7 @@ class Server:
         self.queue: Deque[Request] = collections.deque()
         self.active: List[_Active] = []
         self.done: List[Request] = []
-        self.used = 0.0
-        self.peak_used = 0.0
-        self._resident: Dict[Tuple[int, torch.dtype], bool] = {}
+        self.ledger = HBMLedger(budget_bytes)
         # retired requests' contexts, reused by later requests of the same kind
         self._idle: Dict[Tuple[int, torch.dtype], List["barotr.SWContext"]] = collections.defaultdict(list)
         self.ticks = 0
@@ -112,41 +158,32 @@ class Server:
             raise ValueError(f"request {req.rid} (C{req.nx}) does not fit the HBM budget")
         self.queue.append(req)
 
-    def _need(self, req: Request) -> float:
-        key = (req.nx, req.dtype)
-        if self._idle[key]:
-            return STATE_FIELDS * field_bytes(req.nx, req.dtype)  # workspace already counted
-        extra = 0 if key in self._resident else CONST_PLANES * field_bytes(req.nx, req.dtype)
-        return request_bytes(req.nx, req.dtype) + extra
+    @property
+    def used(self) -> float:
+        return self.ledger.used
 
-    def _drop_idle(self) -> bool:
-        """Free one idle context (largest first); False if there is none."""
-        keys = [k for k, v in self._idle.items() if v]
-        if not keys:
-            return False
-        k = max(keys, key=lambda k: field_bytes(*k))
-        self._idle[k].pop()
-        self.used -= SCRATCH_FIELDS * field_bytes(*k)
-        return True
+    @property
+    def peak_used(self) -> float:
+        return self.ledger.peak
 
     def _admit(self, now: float) -> None:
         while self.queue:
             req = self.queue[0]
-            if self.used + self._need(req) > self.budget:
-                if self._drop_idle():
-                    continue
-                break
-            self.queue.popleft()
-            self.used += self._need(req)
             key = (req.nx, req.dtype)
-            self._resident[key] = True
-            ctx = self._idle[key].pop() if self._idle[key] else self.pool.context(req.nx, req.dtype)
+            if not self.ledger.fits(key):
+                dropped = self.ledger.drop_idle()
+                if dropped is None:
+                    break
+                self._idle[dropped].pop()  # its workspace is freed with the context
+                continue
+            self.queue.popleft()
+            reuse = self.ledger.admit(key)
+            ctx = self._idle[key].pop() if reuse else self.pool.context(req.nx, req.dtype)
             bufs = [ctx.empty(3), ctx.empty(3)]
             for k in range(3):
                 bufs[0][k].copy_(req.init[k].to(device=self.device, dtype=req.dtype))
             req.t_admit = now
             self.active.append(_Active(req, ctx, bufs, 0, req.steps))
-        self.peak_used = max(self.peak_used, self.used)
 
     # -- one tick ---------------------------------------------------------------
     def _retire(self, a: _Active, now: float) -> None:
@@ -154,7 +191,7 @@ class Server:
         a.req.t_done = now
         a.req.checksum = float(h[:, 3:-3, 3:-3].double().sum())
         a.req.result = (h, u, v) if a.req.keep_result else None
-        self.used -= STATE_FIELDS * field_bytes(a.req.nx, a.req.dtype)  # the workspace stays, idle
+        self.ledger.retire((a.req.nx, a.req.dtype))  # the workspace stays, idle
         self._idle[(a.req.nx, a.req.dtype)].append(a.ctx)
         self.done.append(a.req)
 
@@ -220,6 +257,17 @@ def latency_stats(done: Sequence[Request]) -> dict:
             "admission_wait_p99_s": float(np.percentile(wait, 99))}
 
 
+def fcfs_latencies(arrivals: Sequence[float], service: Sequence[float]):
+    """Latencies of a single first-come-first-served server (arrivals sorted)
+    and the time it finishes."""
+    end = 0.0
+    lat = []
+    for t, s in zip(arrivals, service):
+        end = max(end, t) + s
+        lat.append(end - t)
+    return np.array(lat), end
+
+
 def one_at_a_time(requests: Sequence[Request], pool: "ensemble.GridPool", force_fp64: bool = False):
     """The plain path as the comparison: every request run alone (sw_rk2 with
     recorded substep graphs) in arrival order.  Service times are measured;
@@ -238,12 +286,7 @@ def one_at_a_time(requests: Sequence[Request], pool: "ensemble.GridPool", force_
         ctxs[key].rk((h, h, u, v, u, v), r.steps, r.dt, diagnostics=False, graph=True)
         torch.cuda.synchronize(pool.device)
         service.append(time.perf_counter() - t0)
-    end = 0.0
-    lat = []
-    for r, s in zip(requests, service):
-        end = max(end, r.arrival) + s
-        lat.append(end - r.arrival)
-    lat = np.array(lat)
+    lat, end = fcfs_latencies([r.arrival for r in requests], service)
     work = sum(6 * r.nx * r.nx * r.steps for r in requests)
     return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "makespan_s": float(end), "busy_s": float(sum(service)),
