@@ -16,7 +16,7 @@ import os
 import time
 from typing import Dict, Iterable, Optional, Sequence, Tuple
 
-CANDIDATES: Tuple[int, ...] = (8, 12, 16, 24, 32, 48)
+CANDIDATES: Tuple[int, ...] = (8, 12, 16, 20, 24, 26, 28, 30, 32, 36, 40, 48)
 CACHE_ENV = "LICOM_SW_TUNE_CACHE"
 
 
