@@ -197,55 +197,6 @@ __device__ __forceinline__ T vct_at(const Planes<T>& P, const T* u, const T* v, 
 // ---------------------------------------------------------------------------
 // edge lines
 // ---------------------------------------------------------------------------
-// C-grid normal velocity at position r of edge d of face t, before averaging
-// (remap.py:42-55 at the edge rows/columns 3 and N-3).
-template <typename T>
-__device__ T edge_normal_velocity(const Geo& g, const Planes<T>& P, const T* u, const T* v,
-                                  int t, int d, int r) {
-  T w[6];
-  if (d == DN || d == DS) {
-    const int ie = (d == DS) ? 3 : g.N - 3;
-    const int j = 3 + r;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = uct_at(P, u, v, g.at(t, ie - 3 + k, j));
-  } else {
-    const int je = (d == DW) ? 3 : g.N - 3;
-    const int i = 3 + r;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = vct_at(P, u, v, g.at(t, i, je - 3 + k));
-  }
-  return T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
-}
-
-// Upwind mass flux through edge d of face t at position r for the averaged
-// normal velocity vavg (barotr_pure.py:29-49 at the edge line).
-template <typename T>
-__device__ T edge_flux(const Geo& g, const Planes<T>& P, const T* h, int t, int d, int r, T vavg) {
-  T s[6];
-  size_t oc;
-  T dz;
-  if (d == DN || d == DS) {
-    const int ie = (d == DS) ? 3 : g.N - 3;
-    const int j = 3 + r;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] = h[g.at(t, ie - 3 + k, j)];
-    oc = g.at(t, ie, j);
-    dz = dzx(P, oc);
-  } else {
-    const int je = (d == DW) ? 3 : g.N - 3;
-    const int i = 3 + r;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] = h[g.at(t, i, je - 3 + k)];
-    oc = g.at(t, i, je);
-    dz = dzy(P, oc);
-  }
-  const T he = e5(s[0], s[1], s[2], s[3], s[4]);
-  const T hw = w5(s[1], s[2], s[3], s[4], s[5]);
-  const T c = celerity(c6(s[0], s[1], s[2], s[3], s[4], s[5]), dz);
-  const T vc = vavg + T(4.9) / c * (he - hw);
-  return dz * vc + (vc > T(0) ? he : hw) * vc;
-}
-
 template <typename T>
 struct EdgeArgs {
   Geo g;
@@ -259,28 +210,56 @@ struct EdgeArgs {
 
 // Edge values of both faces meeting at a cube edge (communication.py:485-564
 // averaging of the C-grid normal velocity and, on the last stage, of the
-// upwind flux).  Lanes 2k and 2k+1 take the two sides of one edge point and
-// swap their one-sided values, so every cube edge is evaluated once and both
-// faces' entries of evel / eflux (index (t * 4 + d) * nx + r) are written.
-// IEEE addition is commutative, so (own + other) / 2 equals the reference's
-// (mine + nb) / 2 seen from either face.
+// upwind flux).  A group of 8 lanes takes one side of one edge point: lane k
+// (k < 6) evaluates stencil point k across the edge, and the group gathers
+// the six values with shuffles.  Groups 2m and 2m+1 are the two sides of one
+// edge point and swap their one-sided values, so every cube edge is
+// evaluated once and both faces' entries of evel / eflux (index
+// (t * 4 + d) * nx + r) are written.  IEEE addition is commutative, so
+// (own + other) / 2 equals the reference's (mine + nb) / 2 seen from either
+// face.
+constexpr int kEdgeLanes = 8;
+
 template <typename T>
 __device__ __forceinline__ void edge_body(const EdgeArgs<T>& a, int q) {
-  const int nx = a.g.nx;
-  if (q >= 24 * nx) return;  // 24 nx is even: lane pairs stay whole
-  const int p = q >> 1, side = q & 1;
+  const int nx = a.g.nx, N = a.g.N;
+  const int unit = q / kEdgeLanes, k = q % kEdgeLanes;
+  if (unit >= 24 * nx) return;  // 24 nx is even: groups and group pairs stay whole
+  const int p = unit >> 1, side = unit & 1;
   const int e = p / nx, r = p - e * nx;
   const int t = a.ue[e] >> 2, d = a.ue[e] & 3;
   const int tb = a.R.r[t][d][0], db = a.R.r[t][d][1];
   const int rr = a.R.r[t][d][2] ? nx - 1 - r : r;
   const int ts = side ? tb : t, ds = side ? db : d, rs = side ? rr : r;
-  const T vel = edge_normal_velocity(a.g, a.P, a.u, a.v, ts, ds, rs);
-  const T avg = (vel + __shfl_xor(vel, 1)) / T(2.0);
-  const int o = (ts * 4 + ds) * nx + rs;
-  a.evel[o] = avg;
-  if (a.last) {
-    const T f = edge_flux(a.g, a.P, a.h, ts, ds, rs, avg);
-    a.eflux[o] = (f + __shfl_xor(f, 1)) / T(2.0);
+  const bool ns = ds == DN || ds == DS;
+  const int kk = min(k, 5);
+  // stencil point kk across the edge line (remap.py:42-55 at rows / columns 3, N-3)
+  const int line = (ds == DS || ds == DW) ? 3 : N - 3;
+  const size_t o = ns ? a.g.at(ts, line - 3 + kk, 3 + rs) : a.g.at(ts, 3 + rs, line - 3 + kk);
+  const size_t oc = ns ? a.g.at(ts, line, 3 + rs) : a.g.at(ts, 3 + rs, line);
+  const Planes<T>& P = a.P;
+  const T val = ns ? uct_at(P, a.u, a.v, o) : vct_at(P, a.u, a.v, o);
+  const int base = (threadIdx.x & 63) & ~(kEdgeLanes - 1);
+  T w[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) w[m] = __shfl(val, base + m);
+  const T vel = T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
+  const T avg = (vel + __shfl_xor(vel, kEdgeLanes)) / T(2.0);
+  const int oe = (ts * 4 + ds) * nx + rs;
+  if (k == 0) a.evel[oe] = avg;
+  if (a.last) {  // upwind mass flux through the edge (barotr_pure.py:29-49)
+    const T hv = a.h[o];
+    T s6[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) s6[m] = __shfl(hv, base + m);
+    const T dz = ns ? dzx(P, oc) : dzy(P, oc);
+    const T he = e5(s6[0], s6[1], s6[2], s6[3], s6[4]);
+    const T hw = w5(s6[1], s6[2], s6[3], s6[4], s6[5]);
+    const T c = celerity(c6(s6[0], s6[1], s6[2], s6[3], s6[4], s6[5]), dz);
+    const T vc = avg + T(4.9) / c * (he - hw);
+    const T f = dz * vc + (vc > T(0) ? he : hw) * vc;
+    const T fo = __shfl_xor(f, kEdgeLanes);
+    if (k == 0) a.eflux[oe] = (f + fo) / T(2.0);
   }
 }
 
@@ -1740,7 +1719,7 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   StageArgs<T> A;
   int rc = stage_args<T>(c, h, hp, u, v, up, vp, h_out, u_out, v_out, dt, beta, last, A);
   if (rc) return rc;
-  if ((rc = launch(c, K_EDGE, k_edge<T>, dim3(blocks_for(24 * static_cast<size_t>(c->nx))), A.e))) return rc;
+  if ((rc = launch(c, K_EDGE, k_edge<T>, dim3(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))), A.e))) return rc;
   const dim3 ga((A.ma.nwaves + 3) / 4), gb((A.mb.nwaves + 3) / 4), gh(blocks_for(c->plan_n));
   const bool sa = A.ma.hp == A.ma.h, sb = A.mb.up == A.mb.u && A.mb.vp == A.mb.v;
   auto la = [&](auto k) { return launch(c, K_MARCH_A, k, ga, A.ma); };
@@ -2171,7 +2150,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
       if (rc) return rc;
     }
     const StageArgs<T>& A = tab[q];
-    const int blocks[BK_CLASSES] = {static_cast<int>(blocks_for(24 * static_cast<size_t>(c->nx))),
+    const int blocks[BK_CLASSES] = {static_cast<int>(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))),
                                     (A.ma.nwaves + 3) / 4, static_cast<int>(blocks_for(c->plan_n)),
                                     (A.mb.nwaves + 3) / 4};
     for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
