@@ -71,3 +71,11 @@ def test_bad_sizes_are_rejected(lib):
     assert b"nx" in lib.sw_last_error()
     assert lib.sw_create(ctypes.byref(ctx), 48, 7, arr, None) == -1
     assert b"dtype" in lib.sw_last_error()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import sys
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr

@@ -1,0 +1,54 @@
+"""The C768 workload itself (GPU): properties that hold at any size.
+
+The oracle is too slow to step C768 here, so the full-size checks are
+size-independent: graph replay and eager launches give identical bits, the
+band height does not change a bit, the binary64 mass integral is conserved to
+round-off, and the state stays finite.
+"""
+
+from __future__ import annotations
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NX = 768
+SUBSTEPS = 8
+
+
+@pytest.fixture(scope="module")
+def c768():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from licom4py_gpu_amd import barotr, duogrid, initial
+    g = duogrid.grid(NX)
+    h, u, v, sp = initial.seeded_balanced(g)
+    dt = initial.courant_dt(g, h, sp)
+    consts = barotr.as_device_consts(duogrid.barotr_consts(g, 5000.0), torch.device("cuda", 0))
+    core = [torch.as_tensor(x).cuda() for x in initial.state_tuple(h, u, v)[:6]]
+    return barotr.SWContext(consts, torch.float64), core, dt
+
+
+def test_c768_graph_equals_eager_and_band_height_is_invisible(c768):
+    import torch
+    ctx, core, dt = c768
+    ctx.set_tuning(30, 30)
+    eager = [x.clone() for x in ctx.rk(core, SUBSTEPS, dt, graph=False)]
+    graph = ctx.rk(core, SUBSTEPS, dt, graph=True)
+    for k in range(13):
+        assert torch.equal(eager[k], graph[k]), k
+    ctx.set_tuning(16, 44)
+    other = ctx.rk(core, SUBSTEPS, dt, graph=True)
+    for k in range(3):
+        assert torch.equal(eager[k], other[k]), k
+    assert bool(torch.isfinite(eager[0][:, 3:-3, 3:-3]).all())
+
+
+def test_c768_mass_conserved(c768):
+    ctx, core, dt = c768
+    m0, e0 = ctx.integrals(core[0], core[2], core[3])
+    out = ctx.rk(core, SUBSTEPS, dt, diagnostics=False, graph=True)
+    m1, e1 = ctx.integrals(out[0], out[1], out[2])
+    assert abs(m1 - m0) / m0 < 1e-14
+    assert abs(e1 - e0) / e0 < 1e-3
