@@ -5,6 +5,10 @@ Public surface (mirrors the reference's barotropic path):
   schedule.make_schedule_core                              — schedule_core contract
   duogrid.generate / load_npz / barotr_consts              — grid tables, const tuple
   initial.seeded_balanced / w92_case2 / state_tuple        — initial states
+and beyond the reference's single field per process:
+  ensemble.Batch                                           — many fields per launch
+  serving.Server                                           — continuous batching under an HBM budget
+  precision                                                — FP32 vs FP64 report
 The HIP library is loaded lazily on first use (import does not touch the GPU).
 """
 

@@ -1,12 +1,15 @@
 """Project autotuner for the stage kernels' launch shape.
 
 Each wavefront of the two stage kernels marches down a band of R rows of a
-63-column strip; R trades parallelism (more, shorter bands fill more SIMDs)
-against the 6-row stencil warm-up every band repeats.  `tune` times a short
-eager run for every candidate (R_height, R_velocity) on the context's own
-resolution and precision, keeps the fastest, and records it in a JSON cache
-keyed by (nx, dtype, device name).  Results do not depend on R (the kernels
-produce identical bits for any band height), so tuning is free to change it.
+63-column strip; R trades parallelism (more, shorter bands fill more SIMDs,
+and a launch that just fits the wave slots avoids a second round) against the
+6-row stencil warm-up every band repeats.  `tune` times a short eager run
+(per-dispatch hipEvents) for every candidate R of each kernel on the
+context's own resolution and precision, keeps the fastest, and records it in
+a JSON cache keyed by (nx, dtype, device name); `tune_batch` picks one R for
+all members of a batched run by timing the whole batch.  Results do not
+depend on R (the kernels produce identical bits for any band height), so
+tuning is free to change it.
 """
 
 from __future__ import annotations
