@@ -1901,6 +1901,15 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   return SW_OK;
 }
 
+// Recorded graphs bake in the launch shape; drop them when it changes.
+int drop_graphs(sw_ctx* c) {
+  if (c->graphs.empty()) return SW_OK;
+  SW_HIP(hipStreamSynchronize(c->stream));
+  for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
+  c->graphs.clear();
+  return SW_OK;
+}
+
 // Per-context workspace: stage scratch, substep ping-pong fields, edge lines.
 template <typename T>
 int alloc_workspace(sw_ctx* c) {
@@ -2515,9 +2524,10 @@ int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b) {
   if (!ctx) return fail(SW_E_ARG, "ctx is null");
   if (rows_a < 4 || rows_a > 4096 || rows_b < 4 || rows_b > 4096)
     return fail(SW_E_ARG, "rows per wave must be in [4, 4096]");
+  if (rows_a == ctx->rows_a && rows_b == ctx->rows_b) return SW_OK;
   ctx->rows_a = rows_a;
   ctx->rows_b = rows_b;
-  return SW_OK;
+  return drop_graphs(ctx);
 }
 
 int sw_layout_flags(const sw_ctx* ctx, int* flags) {

@@ -107,7 +107,7 @@ def test_stage_matches_golden_fixture():
         _assert_exact(f"golden {name}", _host(gv), want[name])
 
 
-@pytest.mark.parametrize("nx", [8, 13, 48, 70])
+@pytest.mark.parametrize("nx", [8, 13, 48, 70, 140])
 @pytest.mark.parametrize("last", [False, True])
 @pytest.mark.parametrize("beta", [0.0, 1.0])
 def test_stage_matches_oracle(nx, last, beta):
@@ -231,6 +231,22 @@ def test_band_height_does_not_change_results(rows):
     got = ctx.stage(*[_dev(x, torch) for x in core], 20.0, 0.0, True)
     for name, gv, wv in zip(NAMES13, got, want):
         _assert_exact(f"rows={rows} {name}", _host(gv), wv)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_rk2_c140(graph):
+    """A size with three strips per face row and several bands, eager and graph."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(140)
+    ctx = barotr.SWContext(consts, torch.float64)
+    core = [_dev(x, torch) for x in state[:6]]
+    got = ctx.rk(core, 5, dt, graph=graph)
+    want = B.barotr_rk2_core(state, consts, 5, dt)
+    for k, idx in enumerate((0, 2, 3)):
+        _assert_exact(f"C140 graph={graph} field {idx}", _host(got[k]), want[idx])
+    for k in range(10):
+        _assert_exact(f"C140 diagnostic {k}", _host(got[3 + k]), want[6 + k])
 
 
 def _generic_consts(consts, which):
