@@ -49,8 +49,10 @@ enum {
 /* Flags for sw_rk2 / sw_rk3. */
 enum {
   SW_SKIP_DIAG = 1,  /* do not write the 10 diagnostic outputs */
-  SW_GRAPH = 2       /* replay recorded hipGraphs of substep pairs (same
+  SW_GRAPH = 2,      /* replay recorded hipGraphs of substep pairs (same
                         results; the first call per dtb records them) */
+  SW_INPLACE = 4     /* out[0..2] may alias in[] (the inputs are read by the
+                        first substep only): an in-place update */
 };
 
 /* Number of consts and their order: mymodule/schedule.py:95-116

@@ -16,6 +16,7 @@ SW_F64 = 0
 SW_F32 = 1
 SW_SKIP_DIAG = 1
 SW_GRAPH = 2
+SW_INPLACE = 4
 KERNEL_CLASSES = ("edge", "march_a", "halo_h", "march_b", "halo_uv", "diag")
 SW_N_CONSTS = 17
 

@@ -135,16 +135,24 @@ class SWContext:
 
     # -- entry points --------------------------------------------------------
     def rk(self, core6: Sequence[torch.Tensor], nbb: int, dtb: float, rk: int = 2,
-           diagnostics: bool = True, out=None, graph: bool = True):
-        """nbb substeps; returns the 13 output tensors (h, u, v, 10 diagnostics).
-        With ``graph`` the substep pairs replay recorded hipGraphs."""
+           diagnostics: bool = True, out=None, graph: bool = True, inplace: bool = False):
+        """nbb substeps; returns the output tensors (h, u, v, then the 10
+        diagnostics unless ``diagnostics`` is false).  With ``graph`` the
+        substep pairs replay recorded hipGraphs; with ``inplace`` out[0..2]
+        may be the input tensors h, u, v themselves."""
         ins = [self._check_field(x, STATE_KEYS[k]) for k, x in enumerate(core6)]
-        outs = out if out is not None else self.empty(13)
+        outs = out if out is not None else self.empty(13 if diagnostics else 3)
+        if len(outs) < (13 if diagnostics else 3):
+            raise ValueError("out needs 13 tensors with diagnostics, 3 without")
+        for k, x in enumerate(outs):
+            if x.data_ptr() != self._check_field(x, f"out[{k}]").data_ptr():
+                raise ValueError(f"out[{k}] must be contiguous")
         self._bind_stream()
         pin, _k1 = _hip.ptr_array([x.data_ptr() for x in ins])
-        pout, _k2 = _hip.ptr_array([x.data_ptr() for x in outs])
+        pout, _k2 = _hip.ptr_array([x.data_ptr() for x in outs] + [0] * (13 - len(outs)))
         fn = _hip.lib().sw_rk2 if rk == 2 else _hip.lib().sw_rk3
-        flags = (0 if diagnostics else _hip.SW_SKIP_DIAG) | (_hip.SW_GRAPH if graph else 0)
+        flags = ((0 if diagnostics else _hip.SW_SKIP_DIAG) | (_hip.SW_GRAPH if graph else 0)
+                 | (_hip.SW_INPLACE if inplace else 0))
         _hip.check(fn(self._h, pin, pout, int(nbb), float(dtb), flags), f"sw_rk{rk}")
         return outs
 
@@ -250,28 +258,39 @@ def _check_mesh(px: int, py: int) -> None:
             "only px = py = 1 is supported")
 
 
-def _rk_core(state, consts, nbb, dtb, px, py, rk):
+def _rk_core(state, consts, nbb, dtb, px, py, rk, donate=False):
     _check_mesh(px, py)
     if len(state) < N_BAROTR_STATE:
         raise ValueError(f"state must hold {N_BAROTR_STATE} fields, got {len(state)}")
     if nbb == 0:
         return tuple(state[:N_BAROTR_STATE])
     ctx = context_for(consts, state[0].dtype)
+    if donate:
+        # in place: h0, ub, vb and the 10 diagnostic slots of `state` receive the
+        # results; h0p, ubp, vbp are set equal to them (the reference's return)
+        outs = [state[0], state[2], state[3]] + list(state[6:16])
+        ctx.rk(state[:6], nbb, dtb, rk=rk, out=outs, inplace=True)
+        for dst, src in ((1, 0), (4, 2), (5, 3)):
+            if state[dst].data_ptr() != state[src].data_ptr():
+                state[dst].copy_(state[src])
+        return tuple(state[:16]) + tuple(state[16:23])
     out = ctx.rk(state[:6], nbb, dtb, rk=rk)
     h, u, v = out[0], out[1], out[2]
     coupling = tuple(state[16:23])
     return (h, h, u, v, u, v, *out[3:13]) + coupling
 
 
-def barotr_rk2_core(state, consts, nbb, dtb, px=1, py=1):
+def barotr_rk2_core(state, consts, nbb, dtb, px=1, py=1, donate=False):
     """barotr.py:41-61 — nbb RK2-midpoint substeps (stage 1 at dtb/2, stage 2 at
-    dtb from the same *p fields); coupling fields [16:23] pass through."""
-    return _rk_core(state, consts, nbb, dtb, px, py, 2)
+    dtb from the same *p fields); coupling fields [16:23] pass through.  By
+    default returns new tensors; ``donate=True`` updates `state`'s tensors in
+    place and returns them."""
+    return _rk_core(state, consts, nbb, dtb, px, py, 2, donate)
 
 
-def barotr_rk3_core(state, consts, nbb, dtb, px=1, py=1):
+def barotr_rk3_core(state, consts, nbb, dtb, px=1, py=1, donate=False):
     """barotr.py:64-87 — nbb RK3 substeps (dtb/3, dtb/2, dtb; beta = 1)."""
-    return _rk_core(state, consts, nbb, dtb, px, py, 3)
+    return _rk_core(state, consts, nbb, dtb, px, py, 3, donate)
 
 
 def step_rk_logic(h0, h0p, ub, vb, ubp, vbp, consts, dt, beta_d, is_laststep, px=1, py=1):

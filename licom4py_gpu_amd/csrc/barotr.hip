@@ -1760,8 +1760,9 @@ int check_ptrs(const void* const* p, int n, const char* what) {
   return SW_OK;
 }
 
-int check_disjoint(const sw_ctx* c, const void* const in[6], void* const out[13]) {
-  for (int k = 0; k < 13; ++k)
+// out[k] may alias an input for k < first_checked (SW_INPLACE: h, u, v).
+int check_disjoint(const sw_ctx* c, const void* const in[6], void* const out[13], int first_checked = 0) {
+  for (int k = first_checked; k < 13; ++k)
     for (int m = 0; m < 6; ++m)
       if (out[k] && out[k] == in[m]) return fail(SW_E_ARG, "output " + std::to_string(k) + " aliases an input");
   for (int k = 0; k < 13; ++k)
@@ -1845,15 +1846,17 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   // the 10 diagnostic outputs may be null when they are not written
   rc = check_ptrs(const_cast<const void* const*>(out), (flags & SW_SKIP_DIAG) ? 3 : 13, "out");
   if (rc) return rc;
-  rc = check_disjoint(c, in, out);
+  const bool inplace = (flags & SW_INPLACE) != 0;
+  rc = check_disjoint(c, in, out, inplace ? 3 : 0);
   if (rc) return rc;
   if (nbb < 0) return fail(SW_E_ARG, "nbb < 0");
   if (!(dtb == dtb) || dtb > 1e300 || dtb < -1e300) return fail(SW_E_ARG, "dtb must be finite");
   const size_t bytes = c->cells * sizeof(T);
   if (nbb == 0) {
-    SW_HIP(hipMemcpyAsync(out[0], in[0], bytes, hipMemcpyDeviceToDevice, c->stream));
-    SW_HIP(hipMemcpyAsync(out[1], in[2], bytes, hipMemcpyDeviceToDevice, c->stream));
-    SW_HIP(hipMemcpyAsync(out[2], in[3], bytes, hipMemcpyDeviceToDevice, c->stream));
+    const int from[3] = {0, 2, 3};
+    for (int k = 0; k < 3; ++k)
+      if (out[k] != in[from[k]])
+        SW_HIP(hipMemcpyAsync(out[k], in[from[k]], bytes, hipMemcpyDeviceToDevice, c->stream));
     return SW_OK;
   }
   const T* h = static_cast<const T*>(in[0]);
@@ -1870,7 +1873,20 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   const T* du = scratch_field<T>(c, last_stage_scratch + 1);
   const T* dv = scratch_field<T>(c, last_stage_scratch + 2);
 
-  if ((flags & SW_GRAPH) && nbb >= 3 && !c->prof) {
+  if (inplace && !((flags & SW_GRAPH) && nbb >= 3 && !c->prof)) {
+    // outputs may alias inputs: the inputs are read by the first substep only,
+    // so later substeps may write the outputs (the last one does)
+    if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, X))) return rc;
+    T** cur = X;
+    for (int k = 1; k < nbb; ++k) {
+      T** nxt = (k == nbb - 1) ? fin : (cur == X) ? Y : X;
+      if ((rc = run_substep<T>(c, rk, dtb, cur[0], cur[0], cur[1], cur[2], cur[1], cur[2], nxt))) return rc;
+      cur = nxt;
+    }
+    if (cur != fin)
+      for (int k = 0; k < 3; ++k)
+        SW_HIP(hipMemcpyAsync(fin[k], cur[k], bytes, hipMemcpyDeviceToDevice, c->stream));
+  } else if ((flags & SW_GRAPH) && nbb >= 3 && !c->prof) {
     if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, X))) return rc;
     const int pairs = (nbb - 1) / 2;
     int done = 0;

@@ -31,3 +31,112 @@ def make_schedule_core(rk_type_barotr: int = 2, px: int = 1, py: int = 1):
         return s
 
     return schedule_core
+
+
+# ---------------------------------------------------------------------------
+# Chunked driver with asynchronous host snapshots (mymodule/schedule.py:246-313,
+# mymodule/diag.py:45-75)
+# ---------------------------------------------------------------------------
+def global_diag(h0, ub, vb, halo: int = 3) -> dict:
+    """max / min / mean of the interior of h0, ub, vb (diag.py:45-64), from host
+    or device arrays."""
+    import numpy as np
+
+    out = {}
+    for name, a in (("h0", h0), ("ub", ub), ("vb", vb)):
+        a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+        x = a[..., halo:-halo, halo:-halo]
+        out[name] = (float(np.max(x)), float(np.min(x)), float(np.mean(x)))
+    return out
+
+
+def format_diag(tag: str, stats: dict) -> str:
+    """The report layout of diag.py:66-75."""
+    lines = [tag, "                  max                  min                 mean"]
+    for name in ("h0", "ub", "vb"):
+        mx, mn, me = stats[name]
+        lines.append(f"Global {name}: {mx:20.16f}  {mn:20.16f}  {me:20.16f}")
+    lines.append("------------------------------------------------")
+    return "\n".join(lines)
+
+
+class Driver:
+    """Runs `schedule_core` in chunks of baroclinic steps and snapshots (h0, ub,
+    vb) to pinned host memory after each chunk without stopping the device:
+    the copies run on a side stream (default priority) ordered after the
+    chunk by an event, and a worker thread waits for them and calls
+    `on_snapshot(tag, h0, ub, vb)` with numpy arrays (the default prints the
+    reference's diagnostics).  With ``donate`` the chunks update the state in
+    place, and the next chunk waits for the previous snapshot's copies."""
+
+    def __init__(self, state, consts, nbb: int, dtb: float, rk_type_barotr: int = 2,
+                 donate: bool = False, on_snapshot=None):
+        import concurrent.futures
+
+        import torch
+
+        self.state = tuple(state)
+        self.consts = tuple(consts)
+        self.nbb, self.dtb = int(nbb), float(dtb)
+        self.donate = donate
+        self.rk_core = barotr_rk2_core if rk_type_barotr == 2 else barotr_rk3_core
+        self.device = self.state[0].device
+        self.side = torch.cuda.Stream(self.device)
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        self.futures = []
+        self.on_snapshot = on_snapshot or (lambda tag, h, u, v: print(format_diag(tag, global_diag(h, u, v))))
+        self._last_copy = None
+
+    def snapshot(self, tag: str):
+        import torch
+
+        src = (self.state[0], self.state[2], self.state[3])
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(self.device))
+        host = [torch.empty(x.shape, dtype=x.dtype, pin_memory=True) for x in src]
+        with torch.cuda.stream(self.side):
+            self.side.wait_event(ready)
+            for d, s in zip(host, src):
+                d.copy_(s, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(self.side)
+        self._last_copy = done
+
+        def finish(done=done, host=host, src=src):  # src keeps the device tensors alive
+            done.synchronize()
+            self.on_snapshot(tag, *(x.numpy() for x in host))
+
+        self.futures.append(self.pool.submit(finish))
+
+    def _chunk(self, n: int):
+        import torch
+
+        if self.donate and self._last_copy is not None:
+            torch.cuda.current_stream(self.device).wait_event(self._last_copy)
+        s = self.state
+        for _ in range(n):
+            bs = self.rk_core(s[:N_BAROTR_STATE], self.consts[:N_BAROTR_CONSTS], self.nbb, self.dtb,
+                              donate=self.donate)
+            s = tuple(bs) + s[N_BAROTR_STATE:]
+        self.state = s
+
+    def run(self, total_steps: int, diag_interval=None):
+        """`total_steps` baroclinic steps (each nbb barotropic substeps), a
+        snapshot at the start, after every `diag_interval` steps (if given)
+        and at the end (schedule.py:272-313); returns the final state after all
+        snapshots have been handled."""
+        chunk = int(diag_interval) if diag_interval else int(total_steps)
+        self.snapshot("Initial time")
+        done = 0
+        while done < total_steps:
+            n = min(chunk, total_steps - done)
+            self._chunk(n)
+            done += n
+            if diag_interval:
+                self.snapshot(f"Time: step {done}")
+        if not diag_interval:
+            self.snapshot("Final time")
+        for f in self.futures:
+            f.result()
+        self.futures.clear()
+        return self.state

@@ -366,3 +366,57 @@ def test_mass_drift_small(dtype, tol):
     s = barotr.barotr_rk2_core(s, dconsts, 100, dt)
     m1, _ = ctx.integrals(s[0], s[2], s[3])
     assert abs(m1 - m0) / m0 <= tol
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+def test_donate_updates_state_in_place(rk):
+    """donate=True writes the results into the state's own tensors (eager for
+    nbb < 3, substep-pair graphs above) and equals the default call bit for bit;
+    aliased outputs without the in-place flag are refused."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(16)
+    dconsts = barotr.as_device_consts(consts)
+    fn = barotr.barotr_rk2_core if rk == 2 else barotr.barotr_rk3_core
+    for nbb in (1, 2, 3, 6):
+        want = fn(tuple(_dev(x, torch) for x in state), dconsts, nbb, dt)
+        mine = tuple(_dev(x, torch) for x in state)
+        got = fn(mine, dconsts, nbb, dt, donate=True)
+        assert all(g is m for g, m in zip(got, mine))
+        for k in range(16):
+            _assert_exact(f"donate rk{rk} nbb={nbb} {B.STATE_KEYS[k]}", _host(got[k]), _host(want[k]))
+    ctx = barotr.context_for(dconsts, torch.float64)
+    core = [_dev(x, torch) for x in state[:6]]
+    ref = ctx.rk(core, 5, dt, rk=rk, graph=False)
+    outs = [core[0], core[2], core[3]] + ctx.empty(10)
+    ctx.rk(core, 5, dt, rk=rk, out=outs, graph=False, inplace=True)
+    for k in range(13):
+        _assert_exact(f"inplace eager out{k}", _host(outs[k]), _host(ref[k]))
+    with pytest.raises(RuntimeError, match="alias"):
+        ctx.rk(core, 2, dt, rk=rk, out=[core[0], core[2], core[3]] + ctx.empty(10))
+
+
+@pytest.mark.parametrize("donate", [False, True])
+def test_driver_snapshots_match_oracle(donate):
+    """The chunked driver's host snapshots (taken on a side stream while the
+    next chunk runs) equal the oracle's fields after each chunk."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    from licom4py_gpu_amd.schedule import Driver, global_diag
+    state, consts, dt = seeded_state(12)
+    snaps = []
+    drv = Driver(tuple(_dev(x, torch) for x in state), barotr.as_device_consts(consts), 3, dt,
+                 donate=donate, on_snapshot=lambda tag, h, u, v: snaps.append((tag, h.copy(), u.copy(), v.copy())))
+    final = drv.run(5, diag_interval=2)
+    tags = [s[0] for s in snaps]
+    assert tags == ["Initial time", "Time: step 2", "Time: step 4", "Time: step 5"]
+    s = state
+    for i, steps in enumerate((0, 2, 2, 1)):
+        if steps:
+            s = B.schedule_core(s, consts, steps, 3, dt)
+        for name, j, got in (("h0", 0, snaps[i][1]), ("ub", 2, snaps[i][2]), ("vb", 3, snaps[i][3])):
+            _assert_exact(f"snapshot {tags[i]} {name}", got, s[j])
+    for k in range(23):
+        _assert_exact(f"driver final {B.STATE_KEYS[k]}", _host(final[k]), s[k])
+    st = global_diag(final[0], final[2], final[3])
+    assert st["h0"][0] == float(np.max(s[0][..., 3:-3, 3:-3]))

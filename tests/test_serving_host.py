@@ -75,3 +75,16 @@ def test_latency_stats():
                             t_admit=float(k), t_done=float(k) + 1 + k) for k in range(10)]
     st = serving.latency_stats(reqs)
     assert st["n"] == 10 and st["max_s"] == 10.0 and st["p50_s"] == pytest.approx(5.5)
+
+
+def test_global_diag_and_format():
+    from licom4py_gpu_amd.schedule import format_diag, global_diag
+    h = np.zeros((6, 10, 10))
+    h[:, 3:-3, 3:-3] = np.arange(6 * 16).reshape(6, 4, 4)
+    h[0, 0, 0] = 1e9  # halo values are ignored
+    st = global_diag(h, -h, h * 2)
+    assert st["h0"] == (95.0, 0.0, 47.5)
+    assert st["ub"] == (0.0, -95.0, -47.5)
+    text = format_diag("Final time", st)
+    assert text.splitlines()[0] == "Final time"
+    assert "Global h0:" in text and "47.5000000000000000" in text
