@@ -115,10 +115,13 @@ int sw_routing_table(int32_t out[6 * 4 * 3]);
  * (hipExtLaunchKernelGGL); waits for completion.
  * ms[k] / counts[k] = total milliseconds / launches of kernel class k:
  * 0 edge lines, 1 height stage kernel, 2 height halo fill, 3 velocity stage
- * kernel, 4 velocity halo fill, 5 diagnostics.  A measurement helper with no
- * counterpart in the reference. */
+ * kernel, 4 velocity halo fill, 5 diagnostics, 6 fused stage kernel (height
+ * and deep-cell velocity), 7 frame velocity.  Classes 1 and 3 run when fusion
+ * is off (sw_set_tuning rows_f = 0), 6 and 7 when it is on.  A measurement
+ * helper with no counterpart in the reference. */
+#define SW_N_KERNEL_CLASSES 8
 int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
-               double dtb, int reps, double ms[6], int counts[6]);
+               double dtb, int reps, double ms[SW_N_KERNEL_CLASSES], int counts[SW_N_KERNEL_CLASSES]);
 
 /* Global integrals of a state in binary64 with compensated summation over
  * the interior cells, whatever the state precision:
@@ -129,9 +132,12 @@ int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nb
  * interior max / min / mean, mymodule/diag.py:45-75). */
 int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, double out[2]);
 
-/* Rows per wavefront band of the two stage kernels (performance only; any
- * value in [4, 4096] gives the same results). */
-int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b);
+/* Rows per wavefront band of the stage kernels: rows_a / rows_b for the
+ * separate height and velocity kernels, rows_f for the fused kernel, which
+ * runs when rows_f > 0 (the default, 32); rows_f = 0 selects the separate
+ * kernels.  Performance only: every setting in [4, 4096] gives bit-identical
+ * results.  Batched runs (sw_batch_run) use the separate kernels. */
+int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b, int rows_f);
 
 /* Host-only: what sw_create detected in the consts.  Bit 0: dzph_x/dzph_y
  * uniform (read as scalars); bit 1: rdx, rdy, rda, c_dy, d_dx identical on all

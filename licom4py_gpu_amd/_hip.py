@@ -17,7 +17,7 @@ SW_F32 = 1
 SW_SKIP_DIAG = 1
 SW_GRAPH = 2
 SW_INPLACE = 4
-KERNEL_CLASSES = ("edge", "march_a", "halo_h", "march_b", "halo_uv", "diag")
+KERNEL_CLASSES = ("edge", "march_a", "halo_h", "march_b", "halo_uv", "diag", "fused", "frame")
 SW_N_CONSTS = 17
 
 # Every symbol declared in include/licom_sw.h.
@@ -80,7 +80,7 @@ def lib() -> ctypes.CDLL:
     L.sw_profile.restype = ci
     L.sw_integrals.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     L.sw_integrals.restype = ci
-    L.sw_set_tuning.argtypes = [vp, ci, ci]
+    L.sw_set_tuning.argtypes = [vp, ci, ci, ci]
     L.sw_set_tuning.restype = ci
     L.sw_layout_flags.argtypes = [vp, ctypes.POINTER(ci)]
     L.sw_layout_flags.restype = ci

@@ -1,15 +1,16 @@
 """Project autotuner for the stage kernels' launch shape.
 
-Each wavefront of the two stage kernels marches down a band of R rows of a
-63-column strip; R trades parallelism (more, shorter bands fill more SIMDs,
-and a launch that just fits the wave slots avoids a second round) against the
-6-row stencil warm-up every band repeats.  `tune` times a short eager run
-(per-dispatch hipEvents) for every candidate R of each kernel on the
-context's own resolution and precision, keeps the fastest, and records it in
-a JSON cache keyed by (nx, dtype, device name); `tune_batch` picks one R for
-all members of a batched run by timing the whole batch.  Results do not
-depend on R (the kernels produce identical bits for any band height), so
-tuning is free to change it.
+Each wavefront of the stage kernels marches down a band of R rows of a
+column strip; R trades parallelism (more, shorter bands fill more SIMDs, and
+a launch that just fits the wave slots avoids a second round) against the
+stencil warm-up rows every band repeats.  `tune` times a short eager run
+(per-dispatch hipEvents) for every candidate R of the separate height and
+velocity kernels and of the fused kernel (with its frame kernel) on the
+context's own resolution and precision, keeps the fastest band heights and
+the faster of the two paths, and records them in a JSON cache keyed by (nx,
+dtype, device name); `tune_batch` picks one R for all members of a batched
+run by timing the whole batch.  Results do not depend on R or on the path
+(every setting produces identical bits), so tuning is free to change them.
 """
 
 from __future__ import annotations
@@ -21,6 +22,7 @@ from typing import Dict, Iterable, Optional, Sequence, Tuple
 
 CANDIDATES: Tuple[int, ...] = (8, 12, 16, 20, 24, 26, 28, 30, 32, 36, 40, 48)
 CACHE_ENV = "LICOM_SW_TUNE_CACHE"
+_KERNEL_SET = "fused1"  # cache entries of other kernel generations are ignored
 
 
 def _cache_path() -> str:
@@ -28,7 +30,7 @@ def _cache_path() -> str:
 
 
 def _key(nx: int, dtype, device_name: str) -> str:
-    return f"{device_name}|C{nx}|{str(dtype).replace('torch.', '')}"
+    return f"{device_name}|C{nx}|{str(dtype).replace('torch.', '')}|{_KERNEL_SET}"
 
 
 def load_cache() -> Dict[str, dict]:
@@ -52,7 +54,8 @@ def save_cache(cache: Dict[str, dict]) -> None:
 
 def tune(ctx, core6: Sequence, dt: float, substeps: int = 2, reps: int = 2,
          candidates: Iterable[int] = CANDIDATES, use_cache: bool = True) -> dict:
-    """Pick (rows_a, rows_b) for `ctx` by timing; applies and returns it."""
+    """Pick (rows_a, rows_b, rows_f) for `ctx` by timing; applies and
+    returns it (rows_f = 0 when the separate kernels are faster)."""
     import torch
 
     dev_name = torch.cuda.get_device_name(ctx.device)
@@ -60,26 +63,38 @@ def tune(ctx, core6: Sequence, dt: float, substeps: int = 2, reps: int = 2,
     cache = load_cache() if use_cache else {}
     if key in cache:
         best = cache[key]
-        ctx.set_tuning(best["rows_a"], best["rows_b"])
+        ctx.set_tuning(best["rows_a"], best["rows_b"], best["rows_f"])
         return dict(best, cached=True)
     cands = [c for c in candidates if c <= max(4, ctx.nx)]
     times = {}
-    # tune the two kernels independently with the per-class event timer
+
+    def per_launch(prof, *names):
+        return sum(prof[k][0] for k in names) / max(prof[names[0]][1], 1)
+
+    # the separate kernels independently, then the fused kernel + frame, with
+    # the per-class event timer
     for r in cands:
-        ctx.set_tuning(r, r)
+        ctx.set_tuning(r, r, 0)
         ctx.profile(core6, 1, dt)  # warm
-        best_a = best_b = float("inf")
+        best_a = best_b = best_f = float("inf")
         for _ in range(reps):
             prof = ctx.profile(core6, substeps, dt)
-            best_a = min(best_a, prof["march_a"][0] / max(prof["march_a"][1], 1))
-            best_b = min(best_b, prof["march_b"][0] / max(prof["march_b"][1], 1))
-        times[r] = (best_a, best_b)
+            best_a = min(best_a, per_launch(prof, "march_a"))
+            best_b = min(best_b, per_launch(prof, "march_b"))
+        ctx.set_tuning(r, r, r)
+        ctx.profile(core6, 1, dt)
+        for _ in range(reps):
+            prof = ctx.profile(core6, substeps, dt)
+            best_f = min(best_f, per_launch(prof, "fused", "frame"))
+        times[r] = (best_a, best_b, best_f)
     ra = min(cands, key=lambda r: times[r][0])
     rb = min(cands, key=lambda r: times[r][1])
-    ctx.set_tuning(ra, rb)
-    result = {"rows_a": ra, "rows_b": rb,
-              "ms_a": times[ra][0], "ms_b": times[rb][1],
-              "table": {str(r): {"ms_a": t[0], "ms_b": t[1]} for r, t in times.items()},
+    rf = min(cands, key=lambda r: times[r][2])
+    use_fused = times[rf][2] < times[ra][0] + times[rb][1]
+    ctx.set_tuning(ra, rb, rf if use_fused else 0)
+    result = {"rows_a": ra, "rows_b": rb, "rows_f": rf if use_fused else 0,
+              "ms_a": times[ra][0], "ms_b": times[rb][1], "ms_fused_frame": times[rf][2],
+              "table": {str(r): {"ms_a": t[0], "ms_b": t[1], "ms_fused_frame": t[2]} for r, t in times.items()},
               "when": time.strftime("%Y-%m-%d %H:%M:%S")}
     if use_cache:
         cache[key] = result

@@ -78,6 +78,8 @@ class SWContext:
     """Owns one `sw_ctx` (internal const planes, scratch) for a resolution and
     precision.  Launches go to the current torch stream of the call."""
 
+    tuning = (32, 32, 32)  # rows_a, rows_b, rows_f of the library's defaults
+
     def __init__(self, consts: Sequence, dtype: torch.dtype = torch.float64):
         if consts is None:  # filled in by like()
             return
@@ -104,6 +106,7 @@ class SWContext:
         workspace; the device copy of the consts is shared."""
         other = SWContext(None)
         other.nx, other.N, other.dtype, other.device = self.nx, self.N, self.dtype, self.device
+        other.tuning = self.tuning  # sw_create_like copies the band heights
         handle = ctypes.c_void_p()
         _hip.check(_hip.lib().sw_create_like(ctypes.byref(handle), self._h), "sw_create_like")
         other._h = handle
@@ -165,8 +168,8 @@ class SWContext:
         self._bind_stream()
         pin, _k1 = _hip.ptr_array([x.data_ptr() for x in ins])
         pout, _k2 = _hip.ptr_array([x.data_ptr() for x in outs])
-        ms = (ctypes.c_double * 6)()
-        cnt = (ctypes.c_int * 6)()
+        ms = (ctypes.c_double * len(_hip.KERNEL_CLASSES))()
+        cnt = (ctypes.c_int * len(_hip.KERNEL_CLASSES))()
         _hip.check(_hip.lib().sw_profile(self._h, pin, pout, int(nbb), float(dtb), int(reps), ms, cnt),
                    "sw_profile")
         self.last_profile_out = outs  # the pass computes the same outputs as rk()
@@ -182,8 +185,13 @@ class SWContext:
                                            ctypes.c_void_p(v.data_ptr()), out), "sw_integrals")
         return float(out[0]), float(out[1])
 
-    def set_tuning(self, rows_a: int, rows_b: int) -> None:
-        _hip.check(_hip.lib().sw_set_tuning(self._h, int(rows_a), int(rows_b)), "sw_set_tuning")
+    def set_tuning(self, rows_a: int, rows_b: int, rows_f=None) -> None:
+        """Band heights of the separate height / velocity kernels and of the
+        fused kernel (rows_f = 0 selects the separate kernels; None keeps the
+        current value)."""
+        rows_f = self.tuning[2] if rows_f is None else int(rows_f)
+        _hip.check(_hip.lib().sw_set_tuning(self._h, int(rows_a), int(rows_b), rows_f), "sw_set_tuning")
+        self.tuning = (int(rows_a), int(rows_b), rows_f)
 
     def layout_flags(self) -> dict:
         f = ctypes.c_int()

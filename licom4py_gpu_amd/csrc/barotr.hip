@@ -411,26 +411,27 @@ __device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRa
   }
 }
 
-// SAMEP: hp is h (stage 1 of a substep that starts from its own output), so
-// hp at row i-1 is already in the lane's h window and is not loaded.
+// Inputs of one march step: the column point of row crow, the i-face of row
+// i and the cell (and halo column) of row ii.  SAMEP: hp is h (stage 1 of a
+// substep that starts from its own output), so hp at row ii is already in the
+// lane's h window and is not loaded.
 template <typename T, bool GSYM, bool SAMEP>
-__device__ __forceinline__ void a_prefetch(const MarchArgs<T>& a, const WaveTile& w, int i, int hcol,
-                                           APre<T>& p) {
+__device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w, int i, int crow, int ii,
+                                        int hcol, APre<T>& p) {
   const int N = a.g.N, nx = a.g.nx;
   const Planes<T>& P = a.P;
   const uint32_t NS = static_cast<uint32_t>(N);
   const int jl = w.jl, j = w.j, t = w.t;
-  const uint32_t oc = w.fo + static_cast<uint32_t>(min(i + 2, N - 1)) * NS + jl;
+  const uint32_t oc = w.fo + static_cast<uint32_t>(crow) * NS + jl;
   load_ct2<T, GSYM>(a, oc, p.cr);
   p.h = ldg(a.h, oc);
   const uint32_t oi = static_cast<uint32_t>(min(i, N - 1)) * NS + jl;
   p.cdy = ldg(P.cdy, w.mo + oi);
   p.dzx = ldg(P.dzx, P.dz_uniform ? 0 : w.fo + oi);
-  const bool iedge = (i == HALO || i == N - HALO) && j <= N - 4;
+  const bool iedge = (i == HALO || i == N - HALO) && j >= HALO && j <= N - 4;
   const int ie = iedge ? (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO) : 0;
   p.evi = ldg(a.evel, ie);
   p.efi = ldg(a.eflux, ie);
-  const int ii = max(i - 1, 0);
   const uint32_t om = static_cast<uint32_t>(ii) * NS + jl;
   p.rda = ldg(P.rda, w.mo + om);
   p.ddx = ldg(P.ddx, w.mo + om);
@@ -443,6 +444,13 @@ __device__ __forceinline__ void a_prefetch(const MarchArgs<T>& a, const WaveTile
   const uint32_t oh = w.fo + static_cast<uint32_t>(ii) * NS + hcol;
   load_ct2<T, GSYM>(a, oh, p.hr);
   p.hh = ldg(a.h, oh);
+}
+
+// Kernel A's step at face i (marching down): column point row i+2, cell row i-1.
+template <typename T, bool GSYM, bool SAMEP>
+__device__ __forceinline__ void a_prefetch(const MarchArgs<T>& a, const WaveTile& w, int i, int hcol,
+                                           APre<T>& p) {
+  a_fetch<T, GSYM, SAMEP>(a, w, i, min(i + 2, a.g.N - 1), max(i - 1, 0), hcol, p);
 }
 
 // Mass flux through one face (barotr_pure.py:29-49) given the averaged normal
@@ -859,6 +867,474 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Fused stage kernel: kernel A's height march and kernel B's velocity march in
+// one pass over the column loads.
+//
+// A wave covers 64 columns c0 = 57 s .. c0 + 63 of a band and computes the new
+// height on lanes 0..62 (kernel A's expressions).  Kernel B's pressure
+// gradient needs the new height at +-3 rows and columns, so the velocity is
+// written on lanes 3..59 at cells whose 7 x 7 new-height stencil lies inside
+// the face ("deep" cells: rows and columns 6 .. N-7); the 3-cell frame along
+// the face edges needs the new height's halo from the neighbouring faces and
+// is done by k_frame after the halo fill.  Rows of the band: the march covers
+// r0-3 .. r1+2 (clipped to the interior) and writes the height on r0 .. r1-1.
+//
+// Kernel B evaluates, on each i-face and j-face, the same upwind velocity,
+// celerity and face values of h and of the normal velocity as kernel A's mass
+// flux (the same IEEE operations on the same values), so one evaluation per
+// face serves both (face_q).  Every term of the velocity update but the
+// pressure gradient is formed at the height's row ia; those eight partial
+// terms wait in an LDS FIFO for three rows, until the new height at rows
+// ib-3 .. ib+3 of ib = ia - 3 exists, and the update of row ib is completed in
+// the reference's order of operations.
+//
+// Column windows (registers) hold rows i-3 .. i+2 of the current i-face i
+// (slot 5 newest): contravariant u, h and covariant v; the blended new height
+// of the lane's column rows ia-6 .. ia.  LDS per wave: contravariant v and
+// covariant u row buffers (4 rows: own columns written when loaded, at row
+// i+2, halo columns at row ia), the h row buffer (row ia), the blended new
+// height (4 rows) and the FIFO (3 rows).
+// ---------------------------------------------------------------------------
+constexpr int kFStride = 57;  // deep-cell columns per wave and row
+constexpr int kRing = 4;
+constexpr int kFifo = 3;
+constexpr int kFTerms = 8;    // advx advy cor_u cor_v visx visy up vp
+
+template <typename T>
+struct FRing {
+  T v[kRing][kRowBuf];  // contravariant v
+  T u[kRing][kRowBuf];  // covariant u
+  T h[kRowBuf];
+  T b[kRing][kRowBuf];  // blended new height (own columns only)
+  T q[kFifo][kFTerms][64];
+};
+
+// Kernel A's mass flux through a face (barotr_pure.py:29-49) together with
+// what kernel B takes from the same face (remap.py:42-95): s[] = h and w[] =
+// normal velocity samples k-3 .. k+2 across the face, t[] the tangential
+// velocity samples.  F mass flux, uc the upwind-corrected normal velocity, vt
+// the upwinded tangential velocity, C the dissipation coefficient term.
+template <typename T>
+struct FaceQ {
+  T F, uc, vt, C;
+};
+
+template <typename T>
+__device__ __forceinline__ FaceQ<T> face_q(const T* s, const T* w, const T* t, bool edge, T upre_edge, T dz) {
+  const T he = e5(s[0], s[1], s[2], s[3], s[4]);
+  const T hw = w5(s[1], s[2], s[3], s[4], s[5]);
+  const T ue = e5(w[0], w[1], w[2], w[3], w[4]);
+  const T uw = w5(w[1], w[2], w[3], w[4], w[5]);
+  const T inner = T(0.5) * (ue + uw);
+  const T upre = edge ? upre_edge : inner;
+  const T c = celerity(c6(s[0], s[1], s[2], s[3], s[4], s[5]), dz);
+  FaceQ<T> q;
+  q.uc = upre + T(4.9) / c * (he - hw);
+  q.F = dz * q.uc + (q.uc > T(0) ? he : hw) * q.uc;
+  const T te = e5(t[0], t[1], t[2], t[3], t[4]);
+  const T tw = w5(t[1], t[2], t[3], t[4], t[5]);
+  q.vt = upre > T(0) ? te : tw;
+  q.C = T(0.5) * c * (ue - uw);
+  return q;
+}
+
+// Wave of logical block L -> (face, strip, band) of the fused kernel.
+template <typename T>
+__device__ __forceinline__ bool decode_fwave(const MarchArgs<T>& a, int L, WaveTile& w) {
+  const int N = a.g.N;
+  w.lane = threadIdx.x & 63;
+  w.wslot = threadIdx.x >> 6;
+  const int wid = L * 4 + w.wslot;
+  if (wid >= a.nwaves) return false;
+  w.t = wid % 6;
+  const int rest = wid / 6;
+  const int s = rest % a.S;
+  const int band = rest / a.S;
+  w.c0 = kFStride * s;
+  w.j = w.c0 + w.lane;
+  w.jl = min(w.j, N - 1);
+  w.r0 = HALO + band * a.R;
+  w.r1 = min(w.r0 + a.R, N - HALO);
+  const uint32_t NN = static_cast<uint32_t>(N) * N;
+  w.fo = static_cast<uint32_t>(w.t) * NN;
+  w.mo = a.shared ? 0 : w.fo;
+  return true;
+}
+
+// One wave's march over its band.  UP marches from the band's last row to its
+// first, so that neighbouring bands (odd ones up, even ones down) read the
+// rows they share at the same time: both at the start or both at the end of
+// their marches, while the lines are in L2.  Both directions evaluate every
+// expression on the same operands in the same order (the bottom face of a
+// cell is the current face marching down and the previous one marching up).
+template <typename T, bool GSYM, bool SAMEP, bool UP>
+__device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTile& w, FRing<T>& rg) {
+  const int N = a.g.N;
+  const Planes<T>& P = a.P;
+  const int lane = w.lane, j = w.j, c0 = w.c0;
+  const uint32_t NS = static_cast<uint32_t>(N);
+  const T wn = T(1.0) - a.beta, wo = a.beta;
+  // window slots: rows i-3 .. i+2 of the current face i; the newest row
+  // enters at NEW, the cell row ia sits at SA (ia = i - 1 down, i up)
+  constexpr int NEW = UP ? 0 : 5;
+  constexpr int SA = UP ? 3 : 2;
+  constexpr int STEP = UP ? -1 : 1;
+
+  // halo columns of the row buffers: c0-3 .. c0-1 (lanes 0..2), c0+64, c0+65
+  // (lanes 3, 4); a clamped column rewrites the slot of its own column with
+  // the same value
+  int hcol = lane < 3 ? c0 - 3 + lane : c0 + 61 + lane;
+  const bool halo_lane = lane < 5;
+  hcol = halo_lane ? min(max(hcol, 0), N - 1) : w.jl;
+  const int hslot = halo_lane ? hcol - c0 + 3 : lane + 3;
+  const bool own = lane >= 3 && lane < 3 + kFStride && j <= N - 1 - HALO;
+  const bool deep = own && j >= 2 * HALO && j <= N - 1 - 2 * HALO;
+  const int a0 = max(w.r0 - 3, HALO), a1 = min(w.r1 + 3, N - HALO);
+  const int b0 = max(w.r0, 2 * HALO), b1 = min(w.r1, N - 2 * HALO);
+  T* sink = a.sink + threadIdx.x;
+
+  // the step's inputs at face i: column point of the newest row, i-face i,
+  // cell row ia
+  auto fetch = [&](int i, APre<T>& p) {
+    if (UP)
+      a_fetch<T, GSYM, SAMEP>(a, w, i, max(i - 3, 0), min(i, N - 1), hcol, p);
+    else
+      a_fetch<T, GSYM, SAMEP>(a, w, i, min(i + 2, N - 1), max(i - 1, 0), hcol, p);
+  };
+  auto shift = [](T* x) {
+    if (UP) {
+#pragma unroll
+      for (int m = 5; m > 0; --m) x[m] = x[m - 1];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 5; ++m) x[m] = x[m + 1];
+    }
+  };
+
+  T cu[6], hh[6], cv[6], wn7[7];
+  // the first face f0 = a0 (down) / a1 (up) reads rows f0-3 .. f0+2; all but
+  // the newest are loaded here, with the row buffers of the first two cells
+  const int f0 = UP ? a1 : a0;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    const int slot = UP ? m + 1 : m;
+    const int row = f0 - 3 + slot;
+    const uint32_t o = w.fo + static_cast<uint32_t>(row) * NS + w.jl;
+    CtRaw<T> r;
+    load_ct2<T, GSYM>(a, o, r);
+    cu[slot] = ct_u(r);
+    cv[slot] = r.v;
+    hh[slot] = ldg(a.h, o);
+    if (UP ? m < 2 : m >= 2) {
+      rg.v[row & (kRing - 1)][lane + 3] = ct_v(r);
+      rg.u[row & (kRing - 1)][lane + 3] = r.u;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 7; ++m) wn7[m] = T(0);
+  APre<T> pre;
+  fetch(f0, pre);
+
+  // face f0 only: its flux and the velocity terms of the face
+  T Fprev, cdy_prev, ucx_prev, vcx_prev, Cx_prev;
+  {
+    const APre<T> cur = pre;
+    fetch(f0 + STEP, pre);
+    const int rnew = UP ? f0 - 3 : f0 + 2;
+    cu[NEW] = ct_u(cur.cr);
+    cv[NEW] = cur.cr.v;
+    hh[NEW] = cur.h;
+    rg.v[rnew & (kRing - 1)][lane + 3] = ct_v(cur.cr);
+    rg.u[rnew & (kRing - 1)][lane + 3] = cur.cr.u;
+    const bool edge = (f0 == HALO) || (f0 == N - HALO);
+    const FaceQ<T> q = face_q(hh, cu, cv, edge, cur.evi, cur.dzx);
+    Fprev = (a.last && edge) ? cur.efi : q.F;
+    cdy_prev = cur.cdy;
+    ucx_prev = q.uc;
+    vcx_prev = q.vt;
+    Cx_prev = q.C;
+    shift(cu);
+    shift(cv);
+    shift(hh);
+  }
+
+  int sq = 0;  // FIFO slot of rows ia and ib (read before written)
+  for (int n = 1; n <= a1 - a0; ++n) {
+    const int i = f0 + STEP * n;
+    const int ia = UP ? i : i - 1;
+    const int ib = UP ? ia + 3 : ia - 3;
+    const int sa = ia & (kRing - 1), sb = ib & (kRing - 1);
+    const uint32_t om = static_cast<uint32_t>(ia) * NS + w.jl;
+    const uint32_t ob = static_cast<uint32_t>(min(max(ib, 0), N - 1)) * NS + w.jl;
+    // cell inputs of the velocity terms at row ia and the pressure gradient at row ib
+    const T rdx = ldg(P.rdx, w.mo + om), rdy = ldg(P.rdy, w.mo + om), af = ldg(P.af, w.fo + om);
+    T upl = T(0), vpl = T(0);
+    if constexpr (!SAMEP) {
+      upl = ldg(a.up, w.fo + om);
+      vpl = ldg(a.vp, w.fo + om);
+    }
+    const T rdx_b = ldg(P.rdx, w.mo + ob), rdy_b = ldg(P.rdy, w.mo + ob);
+    const APre<T> cur = pre;
+    fetch(i + STEP, pre);
+    const int rnew = UP ? i - 3 : i + 2;
+    cu[NEW] = ct_u(cur.cr);
+    cv[NEW] = cur.cr.v;
+    hh[NEW] = cur.h;
+    rg.v[rnew & (kRing - 1)][lane + 3] = ct_v(cur.cr);
+    rg.u[rnew & (kRing - 1)][lane + 3] = cur.cr.u;
+
+    // i-face at row i: mass flux and velocity terms
+    const bool iedge = (i == HALO) || (i == N - HALO);
+    const FaceQ<T> qi = face_q(hh, cu, cv, iedge, cur.evi, cur.dzx);
+    const T F = (a.last && iedge) ? cur.efi : qi.F;
+
+    // row ia: h, and the halo columns of contravariant v, covariant u and h
+    rg.h[lane + 3] = hh[SA];
+    rg.v[sa][hslot] = ct_v(cur.hr);
+    rg.u[sa][hslot] = cur.hr.u;
+    rg.h[hslot] = cur.hh;
+    wave_lds_sync();
+    T vct_c, ucov_c;
+    FaceQ<T> qj;
+    const bool jedge = (j == HALO) || (j == N - HALO);
+    {
+      T sv[6], su[6], sh[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        sv[m] = rg.v[sa][lane + m];
+        su[m] = rg.u[sa][lane + m];
+        sh[m] = rg.h[lane + m];
+      }
+      vct_c = sv[3];
+      ucov_c = su[3];
+      // j-face at column j, row ia
+      qj = face_q(sh, sv, su, jedge, cur.evj, cur.dzy);
+    }
+    const T G = (a.last && jedge) ? cur.efj : qj.F;
+
+    // the cell's bottom (row ia+1) and top (row ia) i-faces
+    const T Fb = UP ? Fprev : F, Ft = UP ? F : Fprev;
+    const T cdyb = UP ? cdy_prev : cur.cdy, cdyt = UP ? cur.cdy : cdy_prev;
+
+    // height at row ia: divergence (agrid.py:44-53), barotr_pure.py:120
+    const T Gn = __shfl_down(G, 1);
+    const T ddxn = __shfl_down(cur.ddx, 1);
+    const T D = cur.rda * (((Fb * cdyb - Ft * cdyt) + Gn * ddxn) - G * cur.ddx);
+    const T hnew = (SAMEP ? hh[SA] : cur.hp) - D * a.dt;  // hh[SA]: h at row ia
+    const bool arow = ia >= w.r0 && ia < w.r1;
+    T* dh = (own && arow) ? at32(a.h_out, w.fo + static_cast<uint32_t>(ia) * NS + j) : sink;
+    *dh = hnew;
+    const T hb = wn * hnew + wo * hh[SA];
+    rg.b[sa][lane + 3] = hb;
+    if (UP) {
+#pragma unroll
+      for (int m = 6; m > 0; --m) wn7[m] = wn7[m - 1];
+      wn7[0] = hb;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) wn7[m] = wn7[m + 1];
+      wn7[6] = hb;
+    }
+
+    // velocity at row ib: pressure gradient (new height rows ib-3 .. ib+3 and
+    // columns j-3 .. j+3) added to the terms formed three steps ago
+    {
+      T sn[7], tq[kFTerms];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) sn[m] = rg.b[sb][min(lane + m, kRowBuf - 1)];
+#pragma unroll
+      for (int k = 0; k < kFTerms; ++k) tq[k] = rg.q[sq][k][lane];
+      const T hx0 = c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]);
+      const T hx1 = c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
+      const T hy0 = c6(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]);
+      const T hy1 = c6(sn[1], sn[2], sn[3], sn[4], sn[5], sn[6]);
+      const T pgfu = T(-9.80) * (rdx_b * (hx1 - hx0));
+      const T pgfv = T(-9.80) * (rdy_b * (hy1 - hy0));
+      const bool brow = ib >= b0 && ib < b1;
+      const uint32_t oc = w.fo + static_cast<uint32_t>(min(max(ib, 0), N - 1)) * NS + j;
+      T* du = (deep && brow) ? at32(a.u_out, oc) : sink;
+      T* dv = (deep && brow) ? at32(a.v_out, oc) : sink;
+      // barotr_pure.py:133-150 with the terms advx advy cor_u cor_v visx visy up vp
+      *du = tq[6] + (((pgfu + tq[0]) + tq[2]) + tq[4]) * a.dt;
+      *dv = tq[7] + (((pgfv + tq[1]) - tq[3]) + tq[5]) * a.dt;
+    }
+
+    // velocity terms at row ia (barotr_pure.py:133-150, agrid.py:32-41):
+    // vorticity with vv = vb_cx, uu = ub_cy, kinetic-energy gradient,
+    // dissipation and Coriolis
+    {
+      const T ucy_n = __shfl_down(qj.vt, 1);
+      const T vcy_n = __shfl_down(qj.uc, 1);
+      const T Cy_n = __shfl_down(qj.C, 1);
+      const T ucy = qj.vt, vcy = qj.uc, Cy = qj.C;
+      const T ucxb = UP ? ucx_prev : qi.uc, ucxt = UP ? qi.uc : ucx_prev;
+      const T vcxb = UP ? vcx_prev : qi.vt, vcxt = UP ? qi.vt : vcx_prev;
+      const T Cxb = UP ? Cx_prev : qi.C, Cxt = UP ? qi.C : Cx_prev;
+      const T zeta = cur.rda * (((ucy * cur.ddx + vcxb * cdyb) - ucy_n * ddxn) - vcxt * cdyt);
+      const T ku1 = T(0.5) * (ucxb * ucxb + vcxb * vcxb);
+      const T ku0 = T(0.5) * (ucxt * ucxt + vcxt * vcxt);
+      const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
+      const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+      const T uct_c = cu[SA];  // contravariant u at row ia
+      T tq[kFTerms];
+      tq[0] = zeta * vct_c - (ku1 - ku0) * rdx;
+      tq[1] = -zeta * uct_c - (kv1 - kv0) * rdy;
+      tq[2] = af * vct_c;
+      tq[3] = af * uct_c;
+      tq[4] = -(rdx * (Cxb - Cxt));
+      tq[5] = -(rdy * (Cy_n - Cy));
+      tq[6] = SAMEP ? ucov_c : upl;  // u, v at row ia
+      tq[7] = SAMEP ? cv[SA] : vpl;
+#pragma unroll
+      for (int k = 0; k < kFTerms; ++k) rg.q[sq][k][lane] = tq[k];
+      ucx_prev = qi.uc;
+      vcx_prev = qi.vt;
+      Cx_prev = qi.C;
+    }
+    wave_lds_sync();
+
+    Fprev = F;
+    cdy_prev = cur.cdy;
+    sq = sq == kFifo - 1 ? 0 : sq + 1;
+    shift(cu);
+    shift(cv);
+    shift(hh);
+  }
+}
+
+template <typename T, bool GSYM, bool SAMEP>
+__device__ __forceinline__ void fused_body(const MarchArgs<T>& a, int L, FRing<T>* rings) {
+  WaveTile w;
+  if (!decode_fwave(a, L, w)) return;
+  FRing<T>& rg = rings[w.wslot];
+  if ((w.r0 - HALO) / a.R & 1)
+    fused_march<T, GSYM, SAMEP, true>(a, w, rg);
+  else
+    fused_march<T, GSYM, SAMEP, false>(a, w, rg);
+}
+
+template <typename T, bool GSYM, bool SAMEP>
+__global__ __launch_bounds__(256) void k_fused(MarchArgs<T> a) {
+  __shared__ FRing<T> rings[4];
+  fused_body<T, GSYM, SAMEP>(a, xcd_block(), rings);
+}
+
+// Velocity on the frame cells (interior cells within 3 of a face edge, whose
+// pressure-gradient stencil reaches the new height's halo), one thread per
+// cell: kernel B's expressions, in the same order, on stencils gathered from
+// the stage inputs and the halo-filled new height a.hn.
+template <typename T>
+__device__ __forceinline__ T f_evi(const MarchArgs<T>& a, int t, int i, int j) {
+  const int N = a.g.N;
+  const bool e = (i == HALO || i == N - HALO) && j <= N - 4;
+  return ldg(a.evel, e ? (t * 4 + (i == HALO ? DS : DN)) * a.g.nx + (j - HALO) : 0);
+}
+template <typename T>
+__device__ __forceinline__ T f_evj(const MarchArgs<T>& a, int t, int ii, int j) {
+  const int N = a.g.N;
+  const bool e = (j == HALO || j == N - HALO) && ii >= HALO && ii < N - HALO;
+  return ldg(a.evel, e ? (t * 4 + (j == HALO ? DW : DE)) * a.g.nx + (ii - HALO) : 0);
+}
+
+template <typename T, bool GSYM>
+__device__ __forceinline__ void frame_point(const MarchArgs<T>& a, int t, int ii, int j) {
+  const int N = a.g.N;
+  const Planes<T>& P = a.P;
+  const T wn = T(1.0) - a.beta, wo = a.beta;
+  const uint32_t NS = static_cast<uint32_t>(N);
+  const uint32_t fo = static_cast<uint32_t>(t) * NS * NS, mo = a.shared ? 0 : fo;
+  // column j, rows ii-3 .. ii+3; row ii, columns j-3 .. j+3
+  T wu[7], wv[7], wh[7], wb[7], sv[7], su[7], sh[7], sb[7], vct_c = T(0);
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    const uint32_t o = fo + static_cast<uint32_t>(ii - 3 + m) * NS + j;
+    CtRaw<T> r;
+    load_ct2<T, GSYM>(a, o, r);
+    const T h = ldg(a.h, o);
+    wu[m] = (r.g00 * r.u + r.g01 * r.v) * r.sina;
+    wv[m] = r.v;
+    wh[m] = h;
+    wb[m] = wn * ldg(a.hn, o) + wo * h;
+    if (m == 3) vct_c = (r.g10 * r.u + r.g11 * r.v) * r.sina;
+  }
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    const uint32_t o = fo + static_cast<uint32_t>(ii) * NS + (j - 3 + m);
+    CtRaw<T> r;
+    load_ct2<T, GSYM>(a, o, r);
+    const T h = ldg(a.h, o);
+    sv[m] = (r.g10 * r.u + r.g11 * r.v) * r.sina;
+    su[m] = r.u;
+    sh[m] = h;
+    sb[m] = wn * ldg(a.hn, o) + wo * h;
+  }
+  const uint32_t oc = static_cast<uint32_t>(ii) * NS + j;
+  const uint32_t on = static_cast<uint32_t>(ii + 1) * NS + j;
+  T ucx, vcx, Cx, ucx_prev, vcx_prev, Cx_prev, vcy, ucy, Cy, vcy_n, ucy_n, Cy_n;
+  const int i = ii + 1;
+  iface_b(&wu[1], &wv[1], &wh[1], (i == HALO) || (i == N - HALO), f_evi(a, t, i, j),
+          ldg(P.dzx, P.dz_uniform ? 0 : fo + on), ucx, vcx, Cx);
+  iface_b(&wu[0], &wv[0], &wh[0], (ii == HALO) || (ii == N - HALO), f_evi(a, t, ii, j),
+          ldg(P.dzx, P.dz_uniform ? 0 : fo + oc), ucx_prev, vcx_prev, Cx_prev);
+  iface_b(&sv[0], &su[0], &sh[0], (j == HALO) || (j == N - HALO), f_evj(a, t, ii, j),
+          ldg(P.dzy, P.dz_uniform ? 0 : fo + oc), vcy, ucy, Cy);
+  iface_b(&sv[1], &su[1], &sh[1], (j + 1 == HALO) || (j + 1 == N - HALO), f_evj(a, t, ii, j + 1),
+          ldg(P.dzy, P.dz_uniform ? 0 : fo + oc + 1), vcy_n, ucy_n, Cy_n);
+  const T cdy_i = ldg(P.cdy, mo + on), cdy_prev = ldg(P.cdy, mo + oc);
+  const T rda = ldg(P.rda, mo + oc), ddx = ldg(P.ddx, mo + oc), ddx_n = ldg(P.ddx, mo + oc + 1);
+  const T rdx = ldg(P.rdx, mo + oc), rdy = ldg(P.rdy, mo + oc), af = ldg(P.af, fo + oc);
+  const T up = ldg(a.up, fo + oc), vp = ldg(a.vp, fo + oc);
+  const T zeta = rda * (((ucy * ddx + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
+  const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
+  const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
+  const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
+  const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+  const T uct_c = wu[3];
+  const T advx = zeta * vct_c - (ku1 - ku0) * rdx;
+  const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
+  const T visx = -(rdx * (Cx - Cx_prev));
+  const T visy = -(rdy * (Cy_n - Cy));
+  const T hx0 = c6(wb[0], wb[1], wb[2], wb[3], wb[4], wb[5]);
+  const T hx1 = c6(wb[1], wb[2], wb[3], wb[4], wb[5], wb[6]);
+  const T hy0 = c6(sb[0], sb[1], sb[2], sb[3], sb[4], sb[5]);
+  const T hy1 = c6(sb[1], sb[2], sb[3], sb[4], sb[5], sb[6]);
+  const T pgfu = T(-9.80) * (rdx * (hx1 - hx0));
+  const T pgfv = T(-9.80) * (rdy * (hy1 - hy0));
+  *at32(a.u_out, fo + oc) = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
+  *at32(a.v_out, fo + oc) = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
+}
+
+// Frame cells of a face: rows 3..5 and N-6..N-4, then columns 3..5 and
+// N-6..N-4 of rows 6..N-7 (every interior cell when nx <= 6).
+__host__ __device__ inline int frame_cells(int nx) {
+  const int d = nx > 6 ? nx - 6 : 0;
+  return nx * nx - d * d;
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
+  const int nx = a.g.nx, N = a.g.N, per = frame_cells(nx);
+  const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+  if (q >= 6 * per) return;
+  const int t = q / per;
+  int r = q - t * per, ii, j;
+  if (nx <= 6 || r < 3 * nx) {
+    ii = HALO + r / nx;
+    j = HALO + r % nx;
+  } else if (r < 6 * nx) {
+    r -= 3 * nx;
+    ii = N - 6 + r / nx;
+    j = HALO + r % nx;
+  } else {
+    r -= 6 * nx;
+    ii = 2 * HALO + r / 6;
+    const int k = r % 6;
+    j = k < 3 ? HALO + k : N - 9 + k;
+  }
+  frame_point<T, GSYM>(a, t, ii, j);
+}
+
+// ---------------------------------------------------------------------------
 // duo-grid halo fill (update_domain pass 1 + cube_rmp, closed index form)
 // ---------------------------------------------------------------------------
 // Interior cell at distance layer (1..3) inside edge d, along-edge index r.
@@ -1059,7 +1535,7 @@ __global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
 template <typename T>
 struct StageArgs {
   EdgeArgs<T> e;
-  MarchArgs<T> ma, mb;
+  MarchArgs<T> ma, mb, mf;  // kernel A, kernel B, fused kernel and frame
   HaloArgs<T> hh, huv;
 };
 
@@ -1508,6 +1984,7 @@ struct sw_ctx {
   int plan_n = 0;
   int rows_a = 32;         // rows per wave of k_march_a
   int rows_b = 32;         // rows per wave of k_march_b
+  int rows_f = 32;         // rows per wave of k_fused; 0 selects k_march_a + k_march_b
   hipStream_t capture = nullptr;  // private stream used only to record graphs
   struct GraphEntry {
     int rk, pairs;
@@ -1561,7 +2038,8 @@ Planes<T> planes_of(const sw_ctx* c) {
 }
 
 // Kernel classes timed by sw_profile.
-enum { K_EDGE = 0, K_MARCH_A = 1, K_HALO_H = 2, K_MARCH_B = 3, K_HALO_UV = 4, K_DIAG = 5, K_CLASSES = 6 };
+enum { K_EDGE = 0, K_MARCH_A = 1, K_HALO_H = 2, K_MARCH_B = 3, K_HALO_UV = 4, K_DIAG = 5, K_FUSED = 6,
+       K_FRAME = 7, K_CLASSES = 8 };
 
 // Launch `kernel` on the context's stream.  Under sw_profile the launch is
 // repeated prof_reps times (every launch of the step is idempotent) and each
@@ -1705,6 +2183,11 @@ int stage_args(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v,
   A.mb.R = c->rows_b;
   A.mb.nbands = (c->nx + A.mb.R - 1) / A.mb.R;
   A.mb.nwaves = 6 * ma.S * A.mb.nbands;
+  A.mf = ma;
+  A.mf.S = (c->nx + kFStride - 1) / kFStride;
+  A.mf.R = c->rows_f > 0 ? c->rows_f : 32;
+  A.mf.nbands = (c->nx + A.mf.R - 1) / A.mf.R;
+  A.mf.nwaves = 6 * A.mf.S * A.mf.nbands;
   const uint32_t NN = static_cast<uint32_t>(c->N) * static_cast<uint32_t>(c->N);
   A.hh = HaloArgs<T>{plan_of<T>(c), NN, h_out, 1, nullptr, nullptr};
   A.huv = HaloArgs<T>{plan_of<T>(c), NN, nullptr, 1, u_out, v_out};
@@ -1721,6 +2204,21 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   if (rc) return rc;
   if ((rc = launch(c, K_EDGE, k_edge<T>, dim3(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))), A.e))) return rc;
   const dim3 ga((A.ma.nwaves + 3) / 4), gb((A.mb.nwaves + 3) / 4), gh(blocks_for(c->plan_n));
+  if (c->rows_f > 0) {
+    // fused height + deep-cell velocity, new-height halo, frame velocity
+    const bool sf = A.mf.hp == A.mf.h && A.mf.up == A.mf.u && A.mf.vp == A.mf.v;
+    const dim3 gf((A.mf.nwaves + 3) / 4), gr(blocks_for(6 * static_cast<size_t>(frame_cells(c->nx))));
+    auto lf = [&](auto k) { return launch(c, K_FUSED, k, gf, A.mf); };
+    if (c->gsym)
+      rc = sf ? lf(k_fused<T, true, true>) : lf(k_fused<T, true, false>);
+    else
+      rc = sf ? lf(k_fused<T, false, true>) : lf(k_fused<T, false, false>);
+    if (rc) return rc;
+    if ((rc = launch(c, K_HALO_H, k_halo_scalar<T>, gh, A.hh))) return rc;
+    rc = c->gsym ? launch(c, K_FRAME, k_frame<T, true>, gr, A.mf) : launch(c, K_FRAME, k_frame<T, false>, gr, A.mf);
+    if (rc) return rc;
+    return launch(c, K_HALO_UV, k_halo_vector<T>, gh, A.huv);
+  }
   const bool sa = A.ma.hp == A.ma.h, sb = A.mb.up == A.mb.u && A.mb.vp == A.mb.v;
   auto la = [&](auto k) { return launch(c, K_MARCH_A, k, ga, A.ma); };
   auto lb = [&](auto k) { return launch(c, K_MARCH_B, k, gb, A.mb); };
@@ -2399,6 +2897,7 @@ int sw_create_like(sw_ctx** out, const sw_ctx* base) {
   c->gsym = base->gsym;
   c->rows_a = base->rows_a;
   c->rows_b = base->rows_b;
+  c->rows_f = base->rows_f;
   int rc = c->dtype == SW_F64 ? alloc_workspace<double>(c) : alloc_workspace<float>(c);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(c->stream);
@@ -2502,7 +3001,7 @@ int sw_routing_table(int32_t out[6 * 4 * 3]) {
 }
 
 int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb, double dtb,
-               int reps, double ms[6], int counts[6]) {
+               int reps, double ms[SW_N_KERNEL_CLASSES], int counts[SW_N_KERNEL_CLASSES]) {
   if (!ctx || !ms || !counts) return fail(SW_E_ARG, "null argument");
   if (reps < 1 || reps > 1000) return fail(SW_E_ARG, "reps must be in [1, 1000]");
   std::vector<sw_ctx::ProfRec> recs;
@@ -2536,13 +3035,14 @@ int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, doubl
                               : run_integrals<float>(ctx, h, u, v, out);
 }
 
-int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b) {
+int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b, int rows_f) {
   if (!ctx) return fail(SW_E_ARG, "ctx is null");
-  if (rows_a < 4 || rows_a > 4096 || rows_b < 4 || rows_b > 4096)
-    return fail(SW_E_ARG, "rows per wave must be in [4, 4096]");
-  if (rows_a == ctx->rows_a && rows_b == ctx->rows_b) return SW_OK;
+  if (rows_a < 4 || rows_a > 4096 || rows_b < 4 || rows_b > 4096 || (rows_f != 0 && (rows_f < 4 || rows_f > 4096)))
+    return fail(SW_E_ARG, "rows per wave must be in [4, 4096] (rows_f may be 0)");
+  if (rows_a == ctx->rows_a && rows_b == ctx->rows_b && rows_f == ctx->rows_f) return SW_OK;
   ctx->rows_a = rows_a;
   ctx->rows_b = rows_b;
+  ctx->rows_f = rows_f;
   return drop_graphs(ctx);
 }
 

@@ -120,7 +120,7 @@ class GridPool:
                 core = [torch.as_tensor(x).to(device=self.device, dtype=dtype)
                         for x in initial.state_tuple(h, u, v)[:6]]
                 r = autotune.tune(ctx, core, initial.courant_dt(g, h, sp))
-                self.tuning[key] = {"rows_a": r["rows_a"], "rows_b": r["rows_b"]}
+                self.tuning[key] = {"rows_a": r["rows_a"], "rows_b": r["rows_b"], "rows_f": r["rows_f"]}
             self._base[key] = ctx
             return ctx
         return self._base[key].like()
@@ -163,7 +163,7 @@ def main(argv=None) -> int:
     for m in members:  # like() contexts inherit the tuned base; apply per member explicitly
         t = pool.tuning.get((m.ctx.nx, tdt))
         if t:
-            m.ctx.set_tuning(t["rows_a"], t["rows_b"])
+            m.ctx.set_tuning(t["rows_a"], t["rows_b"], t["rows_f"])
     for m in members:
         m.ctx.rk(m.core6, 2, m.dtb, diagnostics=False, graph=True)  # record graphs
     torch.cuda.synchronize()

@@ -183,7 +183,7 @@ def main(argv=None) -> int:
     r.add_argument("--dtype", choices=("f64", "f32"), default="f64")
     r.add_argument("--substeps", type=int, default=10)
     r.add_argument("--copy-mib", type=int, default=1024)
-    r.add_argument("--rows", type=int, nargs=2, default=None)
+    r.add_argument("--rows", type=int, nargs=3, default=None, help="rows_a rows_b rows_f (0: separate kernels)")
     s = sub.add_parser("summarize")
     s.add_argument("db", nargs="+")
     s.add_argument("--out", default=None)

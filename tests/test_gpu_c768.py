@@ -33,15 +33,16 @@ def c768():
 def test_c768_graph_equals_eager_and_band_height_is_invisible(c768):
     import torch
     ctx, core, dt = c768
-    ctx.set_tuning(30, 30)
+    ctx.set_tuning(30, 30, 32)
     eager = [x.clone() for x in ctx.rk(core, SUBSTEPS, dt, graph=False)]
     graph = ctx.rk(core, SUBSTEPS, dt, graph=True)
     for k in range(13):
         assert torch.equal(eager[k], graph[k]), k
-    ctx.set_tuning(16, 44)
-    other = ctx.rk(core, SUBSTEPS, dt, graph=True)
-    for k in range(3):
-        assert torch.equal(eager[k], other[k]), k
+    for rows in ((16, 44, 0), (30, 30, 20), (30, 30, 48)):
+        ctx.set_tuning(*rows)
+        other = ctx.rk(core, SUBSTEPS, dt, graph=True)
+        for k in range(3):
+            assert torch.equal(eager[k], other[k]), (rows, k)
     assert bool(torch.isfinite(eager[0][:, 3:-3, 3:-3]).all())
 
 

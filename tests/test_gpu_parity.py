@@ -107,7 +107,7 @@ def test_stage_matches_golden_fixture():
         _assert_exact(f"golden {name}", _host(gv), want[name])
 
 
-@pytest.mark.parametrize("nx", [8, 13, 48, 70, 140])
+@pytest.mark.parametrize("nx", [7, 8, 13, 48, 57, 58, 70, 140])
 @pytest.mark.parametrize("last", [False, True])
 @pytest.mark.parametrize("beta", [0.0, 1.0])
 def test_stage_matches_oracle(nx, last, beta):
@@ -205,21 +205,31 @@ def test_profile_pass_matches_and_counts():
     state, consts, dt = seeded_state(16)
     ctx = barotr.SWContext(consts, torch.float64)
     core = [_dev(x, torch) for x in state[:6]]
-    prof = ctx.profile(core, 4, dt)
-    assert prof["march_a"][1] == 8 and prof["march_b"][1] == 8 and prof["diag"][1] == 3  # 3 passes
+    prof = ctx.profile(core, 4, dt)  # fused stage kernel (the default)
+    assert prof["fused"][1] == 8 and prof["frame"][1] == 8 and prof["halo_h"][1] == 8
+    assert prof["march_a"][1] == 0 and prof["march_b"][1] == 0 and prof["diag"][1] == 3  # 3 passes
     assert all(ms > 0 for ms, n in prof.values() if n)
     want = ctx.rk(core, 4, dt, graph=False)
     # repeated (idempotent) launches time the kernels and leave the results alone
     prof = ctx.profile(core, 4, dt, reps=3)
-    assert prof["march_b"][1] == 24 and prof["diag"][1] == 9
+    assert prof["fused"][1] == 24 and prof["diag"][1] == 9
+    for k in range(13):
+        assert torch.equal(ctx.last_profile_out[k], want[k]), k
+    ctx.set_tuning(32, 32, 0)  # separate height and velocity kernels
+    prof = ctx.profile(core, 4, dt)
+    assert prof["march_a"][1] == 8 and prof["march_b"][1] == 8 and prof["fused"][1] == 0
     for k in range(13):
         assert torch.equal(ctx.last_profile_out[k], want[k]), k
     flags = ctx.layout_flags()
     assert flags == {"dz_uniform": True, "shared_metrics": True, "gct_symmetric": True}
 
 
-@pytest.mark.parametrize("rows", [(4, 4), (7, 13), (64, 100)])
+@pytest.mark.parametrize("rows", [(4, 4, 0), (7, 13, 0), (64, 100, 0), (4, 4, 4), (7, 7, 5), (8, 8, 13),
+                                  (32, 32, 64), (30, 30, 4096)])
 def test_band_height_does_not_change_results(rows):
+    """Band heights of the separate kernels (rows_f = 0) and of the fused
+    kernel (rows_f > 0, bands shorter and longer than its 6-row stencil reach)
+    give the oracle's bits; nx = 70 puts two strips on a face row."""
     torch = _torch()
     from licom4py_gpu_amd import barotr
     nx = 70
@@ -233,13 +243,15 @@ def test_band_height_does_not_change_results(rows):
         _assert_exact(f"rows={rows} {name}", _host(gv), wv)
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_rk2_c140(graph):
-    """A size with three strips per face row and several bands, eager and graph."""
+@pytest.mark.parametrize("graph,rows_f", [(True, 32), (False, 32), (True, 0), (False, 0)])
+def test_rk2_c140(graph, rows_f):
+    """A size with three strips per face row and several bands, eager and
+    graph, fused stage kernel and separate kernels."""
     torch = _torch()
     from licom4py_gpu_amd import barotr
     state, consts, dt = seeded_state(140)
     ctx = barotr.SWContext(consts, torch.float64)
+    ctx.set_tuning(32, 32, rows_f)
     core = [_dev(x, torch) for x in state[:6]]
     got = ctx.rk(core, 5, dt, graph=graph)
     want = B.barotr_rk2_core(state, consts, 5, dt)
