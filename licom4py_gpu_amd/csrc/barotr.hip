@@ -412,12 +412,12 @@ __device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRa
 }
 
 // Inputs of one march step: the column point of row crow, the i-face of row
-// i and the cell (and halo column) of row ii.  SAMEP: hp is h (stage 1 of a
-// substep that starts from its own output), so hp at row ii is already in the
-// lane's h window and is not loaded.
+// i, the cell of row ii and the halo column of row hrow.  SAMEP: hp is h
+// (stage 1 of a substep that starts from its own output), so hp at row ii is
+// already in the lane's h window and is not loaded.
 template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w, int i, int crow, int ii,
-                                        int hcol, APre<T>& p) {
+                                        int hrow, int hcol, APre<T>& p) {
   const int N = a.g.N, nx = a.g.nx;
   const Planes<T>& P = a.P;
   const uint32_t NS = static_cast<uint32_t>(N);
@@ -441,7 +441,7 @@ __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w
   const int je = jedge ? (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
   p.evj = ldg(a.evel, je);
   p.efj = ldg(a.eflux, je);
-  const uint32_t oh = w.fo + static_cast<uint32_t>(ii) * NS + hcol;
+  const uint32_t oh = w.fo + static_cast<uint32_t>(hrow) * NS + hcol;
   load_ct2<T, GSYM>(a, oh, p.hr);
   p.hh = ldg(a.h, oh);
 }
@@ -450,7 +450,7 @@ __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w
 template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void a_prefetch(const MarchArgs<T>& a, const WaveTile& w, int i, int hcol,
                                            APre<T>& p) {
-  a_fetch<T, GSYM, SAMEP>(a, w, i, min(i + 2, a.g.N - 1), max(i - 1, 0), hcol, p);
+  a_fetch<T, GSYM, SAMEP>(a, w, i, min(i + 2, a.g.N - 1), max(i - 1, 0), max(i - 1, 0), hcol, p);
 }
 
 // Mass flux through one face (barotr_pure.py:29-49) given the averaged normal
@@ -882,30 +882,33 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
 // Kernel B evaluates, on each i-face and j-face, the same upwind velocity,
 // celerity and face values of h and of the normal velocity as kernel A's mass
 // flux (the same IEEE operations on the same values), so one evaluation per
-// face serves both (face_q).  Every term of the velocity update but the
-// pressure gradient is formed at the height's row ia; those eight partial
-// terms wait in an LDS FIFO for three rows, until the new height at rows
-// ib-3 .. ib+3 of ib = ia - 3 exists, and the update of row ib is completed in
-// the reference's order of operations.
+// face serves both (face_q).  Every term of the velocity update is formed at
+// the height's row ia.  The v update completes there: its pressure gradient
+// along j reads the new height of row ia only.  The u update's pressure
+// gradient along i reads the new height at rows ib-3 .. ib+3, so its other
+// terms wait in an LDS FIFO for three rows and the update of row ib (ia - 3
+// marching down, ia + 3 up) is completed in the reference's order of
+// operations.
 //
-// Column windows (registers) hold rows i-3 .. i+2 of the current i-face i
-// (slot 5 newest): contravariant u, h and covariant v; the blended new height
-// of the lane's column rows ia-6 .. ia.  LDS per wave: contravariant v and
-// covariant u row buffers (4 rows: own columns written when loaded, at row
-// i+2, halo columns at row ia), the h row buffer (row ia), the blended new
-// height (4 rows) and the FIFO (3 rows).
+// Column windows (registers) hold rows i-3 .. i+2 of the current i-face i:
+// contravariant u, h and covariant v; and the blended new height of the
+// lane's column over the seven rows the pressure gradient of row ib reads.
+// LDS per wave: row buffers of contravariant v, covariant u and h (4 rows,
+// written with their halo columns when the row is loaded, three rows ahead
+// of its cell row, as the neighbouring strip loads the same lines), the
+// blended new height of row ia and the FIFO (3 rows).
 // ---------------------------------------------------------------------------
 constexpr int kFStride = 57;  // deep-cell columns per wave and row
 constexpr int kRing = 4;
 constexpr int kFifo = 3;
-constexpr int kFTerms = 8;    // advx advy cor_u cor_v visx visy up vp
+constexpr int kFTerms = 5;    // advx cor_u visx u rdx of the u update
 
 template <typename T>
 struct FRing {
   T v[kRing][kRowBuf];  // contravariant v
   T u[kRing][kRowBuf];  // covariant u
-  T h[kRowBuf];
-  T b[kRing][kRowBuf];  // blended new height (own columns only)
+  T h[kRing][kRowBuf];
+  T b[kRowBuf];         // blended new height of row ia (own columns only)
   T q[kFifo][kFTerms][64];
 };
 
@@ -963,10 +966,10 @@ __device__ __forceinline__ bool decode_fwave(const MarchArgs<T>& a, int L, WaveT
 
 // One wave's march over its band.  UP marches from the band's last row to its
 // first, so that neighbouring bands (odd ones up, even ones down) read the
-// rows they share at the same time: both at the start or both at the end of
-// their marches, while the lines are in L2.  Both directions evaluate every
-// expression on the same operands in the same order (the bottom face of a
-// cell is the current face marching down and the previous one marching up).
+// rows they share near the same time, while the lines are in L2.  Both
+// directions evaluate every expression on the same operands in the same order
+// (the bottom face of a cell is the current face marching down and the
+// previous one marching up).
 template <typename T, bool GSYM, bool SAMEP, bool UP>
 __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTile& w, FRing<T>& rg) {
   const int N = a.g.N;
@@ -975,7 +978,8 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   const uint32_t NS = static_cast<uint32_t>(N);
   const T wn = T(1.0) - a.beta, wo = a.beta;
   // window slots: rows i-3 .. i+2 of the current face i; the newest row
-  // enters at NEW, the cell row ia sits at SA (ia = i - 1 down, i up)
+  // (i+2 down, i-3 up: three rows ahead of the cell row) enters at NEW, the
+  // cell row ia (i-1 down, i up) sits at SA
   constexpr int NEW = UP ? 0 : 5;
   constexpr int SA = UP ? 3 : 2;
   constexpr int STEP = UP ? -1 : 1;
@@ -993,13 +997,11 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   const int b0 = max(w.r0, 2 * HALO), b1 = min(w.r1, N - 2 * HALO);
   T* sink = a.sink + threadIdx.x;
 
-  // the step's inputs at face i: column point of the newest row, i-face i,
-  // cell row ia
+  // the step's inputs at face i: the newest row's column point and halo
+  // column, i-face i, cell row ia
   auto fetch = [&](int i, APre<T>& p) {
-    if (UP)
-      a_fetch<T, GSYM, SAMEP>(a, w, i, max(i - 3, 0), min(i, N - 1), hcol, p);
-    else
-      a_fetch<T, GSYM, SAMEP>(a, w, i, min(i + 2, N - 1), max(i - 1, 0), hcol, p);
+    const int rn = UP ? max(i - 3, 0) : min(i + 2, N - 1);
+    a_fetch<T, GSYM, SAMEP>(a, w, i, rn, UP ? min(i, N - 1) : max(i - 1, 0), rn, hcol, p);
   };
   auto shift = [](T* x) {
     if (UP) {
@@ -1010,24 +1012,35 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       for (int m = 0; m < 5; ++m) x[m] = x[m + 1];
     }
   };
+  // row buffers of the newest row: contravariant v, covariant u and h with
+  // the halo columns
+  auto put_row = [&](int row, const CtRaw<T>& c, T hc, const CtRaw<T>& hr, T hh_) {
+    const int s = row & (kRing - 1);
+    rg.v[s][lane + 3] = ct_v(c);
+    rg.u[s][lane + 3] = c.u;
+    rg.h[s][lane + 3] = hc;
+    rg.v[s][hslot] = ct_v(hr);
+    rg.u[s][hslot] = hr.u;
+    rg.h[s][hslot] = hh_;
+  };
 
   T cu[6], hh[6], cv[6], wn7[7];
   // the first face f0 = a0 (down) / a1 (up) reads rows f0-3 .. f0+2; all but
-  // the newest are loaded here, with the row buffers of the first two cells
+  // the newest are loaded here, with the row buffers of the first cells
   const int f0 = UP ? a1 : a0;
 #pragma unroll
   for (int m = 0; m < 5; ++m) {
     const int slot = UP ? m + 1 : m;
     const int row = f0 - 3 + slot;
-    const uint32_t o = w.fo + static_cast<uint32_t>(row) * NS + w.jl;
-    CtRaw<T> r;
-    load_ct2<T, GSYM>(a, o, r);
+    const uint32_t o = w.fo + static_cast<uint32_t>(row) * NS;
+    CtRaw<T> r, hr;
+    load_ct2<T, GSYM>(a, o + w.jl, r);
     cu[slot] = ct_u(r);
     cv[slot] = r.v;
-    hh[slot] = ldg(a.h, o);
+    hh[slot] = ldg(a.h, o + w.jl);
     if (UP ? m < 2 : m >= 2) {
-      rg.v[row & (kRing - 1)][lane + 3] = ct_v(r);
-      rg.u[row & (kRing - 1)][lane + 3] = r.u;
+      load_ct2<T, GSYM>(a, o + hcol, hr);
+      put_row(row, r, hh[slot], hr, ldg(a.h, o + hcol));
     }
   }
 #pragma unroll
@@ -1040,12 +1053,10 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   {
     const APre<T> cur = pre;
     fetch(f0 + STEP, pre);
-    const int rnew = UP ? f0 - 3 : f0 + 2;
     cu[NEW] = ct_u(cur.cr);
     cv[NEW] = cur.cr.v;
     hh[NEW] = cur.h;
-    rg.v[rnew & (kRing - 1)][lane + 3] = ct_v(cur.cr);
-    rg.u[rnew & (kRing - 1)][lane + 3] = cur.cr.u;
+    put_row(UP ? f0 - 3 : f0 + 2, cur.cr, cur.h, cur.hr, cur.hh);
     const bool edge = (f0 == HALO) || (f0 == N - HALO);
     const FaceQ<T> q = face_q(hh, cu, cv, edge, cur.evi, cur.dzx);
     Fprev = (a.last && edge) ? cur.efi : q.F;
@@ -1063,37 +1074,28 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     const int i = f0 + STEP * n;
     const int ia = UP ? i : i - 1;
     const int ib = UP ? ia + 3 : ia - 3;
-    const int sa = ia & (kRing - 1), sb = ib & (kRing - 1);
+    const int sa = ia & (kRing - 1);
     const uint32_t om = static_cast<uint32_t>(ia) * NS + w.jl;
-    const uint32_t ob = static_cast<uint32_t>(min(max(ib, 0), N - 1)) * NS + w.jl;
-    // cell inputs of the velocity terms at row ia and the pressure gradient at row ib
+    // cell inputs of the velocity terms at row ia
     const T rdx = ldg(P.rdx, w.mo + om), rdy = ldg(P.rdy, w.mo + om), af = ldg(P.af, w.fo + om);
     T upl = T(0), vpl = T(0);
     if constexpr (!SAMEP) {
       upl = ldg(a.up, w.fo + om);
       vpl = ldg(a.vp, w.fo + om);
     }
-    const T rdx_b = ldg(P.rdx, w.mo + ob), rdy_b = ldg(P.rdy, w.mo + ob);
     const APre<T> cur = pre;
     fetch(i + STEP, pre);
-    const int rnew = UP ? i - 3 : i + 2;
     cu[NEW] = ct_u(cur.cr);
     cv[NEW] = cur.cr.v;
     hh[NEW] = cur.h;
-    rg.v[rnew & (kRing - 1)][lane + 3] = ct_v(cur.cr);
-    rg.u[rnew & (kRing - 1)][lane + 3] = cur.cr.u;
+    put_row(UP ? i - 3 : i + 2, cur.cr, cur.h, cur.hr, cur.hh);
 
     // i-face at row i: mass flux and velocity terms
     const bool iedge = (i == HALO) || (i == N - HALO);
     const FaceQ<T> qi = face_q(hh, cu, cv, iedge, cur.evi, cur.dzx);
     const T F = (a.last && iedge) ? cur.efi : qi.F;
 
-    // row ia: h, and the halo columns of contravariant v, covariant u and h
-    rg.h[lane + 3] = hh[SA];
-    rg.v[sa][hslot] = ct_v(cur.hr);
-    rg.u[sa][hslot] = cur.hr.u;
-    rg.h[hslot] = cur.hh;
-    wave_lds_sync();
+    // j-face at column j, row ia (its row buffers were written three steps ago)
     T vct_c, ucov_c;
     FaceQ<T> qj;
     const bool jedge = (j == HALO) || (j == N - HALO);
@@ -1103,11 +1105,10 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       for (int m = 0; m < 6; ++m) {
         sv[m] = rg.v[sa][lane + m];
         su[m] = rg.u[sa][lane + m];
-        sh[m] = rg.h[lane + m];
+        sh[m] = rg.h[sa][lane + m];
       }
       vct_c = sv[3];
       ucov_c = su[3];
-      // j-face at column j, row ia
       qj = face_q(sh, sv, su, jedge, cur.evj, cur.dzy);
     }
     const T G = (a.last && jedge) ? cur.efj : qj.F;
@@ -1125,7 +1126,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     T* dh = (own && arow) ? at32(a.h_out, w.fo + static_cast<uint32_t>(ia) * NS + j) : sink;
     *dh = hnew;
     const T hb = wn * hnew + wo * hh[SA];
-    rg.b[sa][lane + 3] = hb;
+    rg.b[lane + 3] = hb;
     if (UP) {
 #pragma unroll
       for (int m = 6; m > 0; --m) wn7[m] = wn7[m - 1];
@@ -1136,65 +1137,71 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       wn7[6] = hb;
     }
 
-    // velocity at row ib: pressure gradient (new height rows ib-3 .. ib+3 and
-    // columns j-3 .. j+3) added to the terms formed three steps ago
+    // velocity terms at row ia (barotr_pure.py:133-150, agrid.py:32-41):
+    // vorticity with vv = vb_cx, uu = ub_cy, kinetic-energy gradient,
+    // dissipation and Coriolis
+    const T ucy_n = __shfl_down(qj.vt, 1);
+    const T vcy_n = __shfl_down(qj.uc, 1);
+    const T Cy_n = __shfl_down(qj.C, 1);
+    const T ucy = qj.vt, vcy = qj.uc, Cy = qj.C;
+    const T ucxb = UP ? ucx_prev : qi.uc, ucxt = UP ? qi.uc : ucx_prev;
+    const T vcxb = UP ? vcx_prev : qi.vt, vcxt = UP ? qi.vt : vcx_prev;
+    const T Cxb = UP ? Cx_prev : qi.C, Cxt = UP ? qi.C : Cx_prev;
+    const T zeta = cur.rda * (((ucy * cur.ddx + vcxb * cdyb) - ucy_n * ddxn) - vcxt * cdyt);
+    const T ku1 = T(0.5) * (ucxb * ucxb + vcxb * vcxb);
+    const T ku0 = T(0.5) * (ucxt * ucxt + vcxt * vcxt);
+    const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
+    const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+    const T uct_c = cu[SA];  // contravariant u at row ia
+    const T advx = zeta * vct_c - (ku1 - ku0) * rdx;
+    const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
+    const T visx = -(rdx * (Cxb - Cxt));
+    const T visy = -(rdy * (Cy_n - Cy));
+    wave_lds_sync();
+
+    // v at row ia: the pressure gradient along j needs the new height of row
+    // ia only (columns j-3 .. j+3)
     {
-      T sn[7], tq[kFTerms];
+      T sn[7];
 #pragma unroll
-      for (int m = 0; m < 7; ++m) sn[m] = rg.b[sb][min(lane + m, kRowBuf - 1)];
+      for (int m = 0; m < 7; ++m) sn[m] = rg.b[min(lane + m, kRowBuf - 1)];
+      const T hy0 = c6(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]);
+      const T hy1 = c6(sn[1], sn[2], sn[3], sn[4], sn[5], sn[6]);
+      const T pgfv = T(-9.80) * (rdy * (hy1 - hy0));
+      const bool vrow = ia >= b0 && ia < b1;
+      T* dv = (deep && vrow) ? at32(a.v_out, w.fo + static_cast<uint32_t>(ia) * NS + j) : sink;
+      *dv = (SAMEP ? cv[SA] : vpl) + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
+    }
+
+    // u at row ib: the pressure gradient along i needs the new height of rows
+    // ib-3 .. ib+3; the other terms were formed three steps ago
+    {
+      T tq[kFTerms];
 #pragma unroll
       for (int k = 0; k < kFTerms; ++k) tq[k] = rg.q[sq][k][lane];
       const T hx0 = c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]);
       const T hx1 = c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
-      const T hy0 = c6(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]);
-      const T hy1 = c6(sn[1], sn[2], sn[3], sn[4], sn[5], sn[6]);
-      const T pgfu = T(-9.80) * (rdx_b * (hx1 - hx0));
-      const T pgfv = T(-9.80) * (rdy_b * (hy1 - hy0));
-      const bool brow = ib >= b0 && ib < b1;
+      const T pgfu = T(-9.80) * (tq[4] * (hx1 - hx0));
+      const bool urow = ib >= b0 && ib < b1;
       const uint32_t oc = w.fo + static_cast<uint32_t>(min(max(ib, 0), N - 1)) * NS + j;
-      T* du = (deep && brow) ? at32(a.u_out, oc) : sink;
-      T* dv = (deep && brow) ? at32(a.v_out, oc) : sink;
-      // barotr_pure.py:133-150 with the terms advx advy cor_u cor_v visx visy up vp
-      *du = tq[6] + (((pgfu + tq[0]) + tq[2]) + tq[4]) * a.dt;
-      *dv = tq[7] + (((pgfv + tq[1]) - tq[3]) + tq[5]) * a.dt;
-    }
-
-    // velocity terms at row ia (barotr_pure.py:133-150, agrid.py:32-41):
-    // vorticity with vv = vb_cx, uu = ub_cy, kinetic-energy gradient,
-    // dissipation and Coriolis
-    {
-      const T ucy_n = __shfl_down(qj.vt, 1);
-      const T vcy_n = __shfl_down(qj.uc, 1);
-      const T Cy_n = __shfl_down(qj.C, 1);
-      const T ucy = qj.vt, vcy = qj.uc, Cy = qj.C;
-      const T ucxb = UP ? ucx_prev : qi.uc, ucxt = UP ? qi.uc : ucx_prev;
-      const T vcxb = UP ? vcx_prev : qi.vt, vcxt = UP ? qi.vt : vcx_prev;
-      const T Cxb = UP ? Cx_prev : qi.C, Cxt = UP ? qi.C : Cx_prev;
-      const T zeta = cur.rda * (((ucy * cur.ddx + vcxb * cdyb) - ucy_n * ddxn) - vcxt * cdyt);
-      const T ku1 = T(0.5) * (ucxb * ucxb + vcxb * vcxb);
-      const T ku0 = T(0.5) * (ucxt * ucxt + vcxt * vcxt);
-      const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
-      const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
-      const T uct_c = cu[SA];  // contravariant u at row ia
-      T tq[kFTerms];
-      tq[0] = zeta * vct_c - (ku1 - ku0) * rdx;
-      tq[1] = -zeta * uct_c - (kv1 - kv0) * rdy;
-      tq[2] = af * vct_c;
-      tq[3] = af * uct_c;
-      tq[4] = -(rdx * (Cxb - Cxt));
-      tq[5] = -(rdy * (Cy_n - Cy));
-      tq[6] = SAMEP ? ucov_c : upl;  // u, v at row ia
-      tq[7] = SAMEP ? cv[SA] : vpl;
+      T* du = (deep && urow) ? at32(a.u_out, oc) : sink;
+      // barotr_pure.py:133-150 with the terms advx, cor_u, visx, u of row ib
+      *du = tq[3] + (((pgfu + tq[0]) + tq[1]) + tq[2]) * a.dt;
+      tq[0] = advx;
+      tq[1] = af * vct_c;
+      tq[2] = visx;
+      tq[3] = SAMEP ? ucov_c : upl;  // u at row ia
+      tq[4] = rdx;
 #pragma unroll
       for (int k = 0; k < kFTerms; ++k) rg.q[sq][k][lane] = tq[k];
-      ucx_prev = qi.uc;
-      vcx_prev = qi.vt;
-      Cx_prev = qi.C;
     }
     wave_lds_sync();
 
     Fprev = F;
     cdy_prev = cur.cdy;
+    ucx_prev = qi.uc;
+    vcx_prev = qi.vt;
+    Cx_prev = qi.C;
     sq = sq == kFifo - 1 ? 0 : sq + 1;
     shift(cu);
     shift(cv);
