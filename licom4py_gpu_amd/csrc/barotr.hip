@@ -1243,102 +1243,155 @@ __device__ __forceinline__ T f_evj(const MarchArgs<T>& a, int t, int ii, int j) 
   return ldg(a.evel, e ? (t * 4 + (j == HALO ? DW : DE)) * a.g.nx + (ii - HALO) : 0);
 }
 
+// Frame tiles of a face: the two groups of frame rows (3..5, N-6..N-4) in
+// chunks of kTile columns, then the two groups of frame columns (3..5,
+// N-6..N-4) of rows 6..N-7 in chunks of kTile rows (nx >= 7).  A block loads
+// the points its cells' stencils read (the cells and 3 around them) into LDS
+// with row-contiguous loads.  Then a group of 4 lanes takes each cell: lane k
+// evaluates face k of the cell (0 bottom i-face, 1 top i-face, 2 the j-face
+// at j, 3 the j-face at j+1; kernel B's expressions) and even / odd lanes the
+// pressure gradient along i / j, and lane 0 gathers them and completes the
+// update in the reference's order of operations.
+constexpr int kTile = 21;                    // cells along a tile's long side
+constexpr int kTilePts = (kTile + 6) * 9;    // points of a tile's region
+constexpr int kFrameLanes = 4;
+
+__host__ __device__ inline int frame_tiles(int nx) {
+  return 2 * ((nx + kTile - 1) / kTile) + 2 * ((nx - 6 + kTile - 1) / kTile);
+}
+
 template <typename T, bool GSYM>
-__device__ __forceinline__ void frame_point(const MarchArgs<T>& a, int t, int ii, int j) {
-  const int N = a.g.N;
+__global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
+  __shared__ T s[9][kTilePts];  // u v g00 g01 g10 g11 sina h hn
+  const int nx = a.g.nx, N = a.g.N;
+  const int nr = (nx + kTile - 1) / kTile, ns = (nx - 6 + kTile - 1) / kTile;
+  const int per = 2 * nr + 2 * ns;
+  const int t = static_cast<int>(blockIdx.x) / per;
+  int k = static_cast<int>(blockIdx.x) - t * per, cr0, crows, cc0, ccols;
+  if (k < 2 * nr) {
+    const int g = k / nr, c = k - g * nr;
+    cr0 = g ? N - 2 * HALO : HALO;
+    crows = 3;
+    cc0 = HALO + kTile * c;
+    ccols = min(kTile, nx - kTile * c);
+  } else {
+    k -= 2 * nr;
+    const int g = k / ns, c = k - g * ns;
+    cr0 = 2 * HALO + kTile * c;
+    crows = min(kTile, nx - 6 - kTile * c);
+    cc0 = g ? N - 2 * HALO : HALO;
+    ccols = 3;
+  }
+  const int ra = cr0 - 3, ca = cc0 - 3, rcols = ccols + 6, npts = (crows + 6) * rcols;
   const Planes<T>& P = a.P;
-  const T wn = T(1.0) - a.beta, wo = a.beta;
   const uint32_t NS = static_cast<uint32_t>(N);
   const uint32_t fo = static_cast<uint32_t>(t) * NS * NS, mo = a.shared ? 0 : fo;
-  // column j, rows ii-3 .. ii+3; row ii, columns j-3 .. j+3
-  T wu[7], wv[7], wh[7], wb[7], sv[7], su[7], sh[7], sb[7], vct_c = T(0);
-#pragma unroll
-  for (int m = 0; m < 7; ++m) {
-    const uint32_t o = fo + static_cast<uint32_t>(ii - 3 + m) * NS + j;
-    CtRaw<T> r;
-    load_ct2<T, GSYM>(a, o, r);
-    const T h = ldg(a.h, o);
-    wu[m] = (r.g00 * r.u + r.g01 * r.v) * r.sina;
-    wv[m] = r.v;
-    wh[m] = h;
-    wb[m] = wn * ldg(a.hn, o) + wo * h;
-    if (m == 3) vct_c = (r.g10 * r.u + r.g11 * r.v) * r.sina;
+  // the cell of this lane's group (the last cell for lanes past the tile's
+  // cells, which only help fill the tile)
+  const int q0 = static_cast<int>(threadIdx.x) / kFrameLanes, kl = static_cast<int>(threadIdx.x) % kFrameLanes;
+  const int q = min(q0, crows * ccols - 1);
+  const int ii = cr0 + q / ccols, j = cc0 + q % ccols;
+  const int li = ii - ra, lj = j - ca;
+  const bool iface = kl < 2;
+  const int fr = kl == 0 ? ii + 1 : ii, fc = kl == 3 ? j + 1 : j;
+  const bool pi = (kl & 1) == 0;
+  const uint32_t oc = static_cast<uint32_t>(ii) * NS + static_cast<uint32_t>(j);
+  // global inputs besides the tile, issued before the tile loads
+  const T fev = iface ? f_evi(a, t, fr, j) : f_evj(a, t, ii, fc);
+  const uint32_t fd = static_cast<uint32_t>(fr) * NS + static_cast<uint32_t>(fc);
+  const T fdz = iface ? ldg(P.dzx, P.dz_uniform ? 0 : fo + fd) : ldg(P.dzy, P.dz_uniform ? 0 : fo + fd);
+  const T rpg = pi ? ldg(P.rdx, mo + oc) : ldg(P.rdy, mo + oc);
+  // cell inputs of lane 0's update (lane kl loads the kl-th group)
+  const uint32_t on = oc + NS;
+  T cin[4];
+  if (kl == 0) {
+    cin[0] = ldg(P.cdy, mo + on); cin[1] = ldg(P.cdy, mo + oc); cin[2] = ldg(P.rda, mo + oc); cin[3] = ldg(P.ddx, mo + oc);
+  } else if (kl == 1) {
+    cin[0] = ldg(P.ddx, mo + oc + 1); cin[1] = ldg(P.rdx, mo + oc); cin[2] = ldg(P.rdy, mo + oc); cin[3] = ldg(P.af, fo + oc);
+  } else {
+    cin[0] = ldg(a.up, fo + oc); cin[1] = ldg(a.vp, fo + oc); cin[2] = T(0); cin[3] = T(0);
   }
-#pragma unroll
-  for (int m = 0; m < 7; ++m) {
-    const uint32_t o = fo + static_cast<uint32_t>(ii) * NS + (j - 3 + m);
-    CtRaw<T> r;
-    load_ct2<T, GSYM>(a, o, r);
-    const T h = ldg(a.h, o);
-    sv[m] = (r.g10 * r.u + r.g11 * r.v) * r.sina;
-    su[m] = r.u;
-    sh[m] = h;
-    sb[m] = wn * ldg(a.hn, o) + wo * h;
+  for (int e = static_cast<int>(threadIdx.x); e < npts; e += 256) {
+    const int rr = e / rcols, cc = e - rr * rcols;
+    const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
+    s[0][e] = ldg(a.u, o);
+    s[1][e] = ldg(a.v, o);
+    s[2][e] = ldg(P.g00, o);
+    s[3][e] = ldg(P.g01, o);
+    if (!GSYM) {
+      s[4][e] = ldg(P.g10, o);
+      s[5][e] = ldg(P.g11, o);
+    }
+    s[6][e] = ldg(P.sina, o);
+    s[7][e] = ldg(a.h, o);
+    s[8][e] = ldg(a.hn, o);
   }
-  const uint32_t oc = static_cast<uint32_t>(ii) * NS + j;
-  const uint32_t on = static_cast<uint32_t>(ii + 1) * NS + j;
-  T ucx, vcx, Cx, ucx_prev, vcx_prev, Cx_prev, vcy, ucy, Cy, vcy_n, ucy_n, Cy_n;
-  const int i = ii + 1;
-  iface_b(&wu[1], &wv[1], &wh[1], (i == HALO) || (i == N - HALO), f_evi(a, t, i, j),
-          ldg(P.dzx, P.dz_uniform ? 0 : fo + on), ucx, vcx, Cx);
-  iface_b(&wu[0], &wv[0], &wh[0], (ii == HALO) || (ii == N - HALO), f_evi(a, t, ii, j),
-          ldg(P.dzx, P.dz_uniform ? 0 : fo + oc), ucx_prev, vcx_prev, Cx_prev);
-  iface_b(&sv[0], &su[0], &sh[0], (j == HALO) || (j == N - HALO), f_evj(a, t, ii, j),
-          ldg(P.dzy, P.dz_uniform ? 0 : fo + oc), vcy, ucy, Cy);
-  iface_b(&sv[1], &su[1], &sh[1], (j + 1 == HALO) || (j + 1 == N - HALO), f_evj(a, t, ii, j + 1),
-          ldg(P.dzy, P.dz_uniform ? 0 : fo + oc + 1), vcy_n, ucy_n, Cy_n);
-  const T cdy_i = ldg(P.cdy, mo + on), cdy_prev = ldg(P.cdy, mo + oc);
-  const T rda = ldg(P.rda, mo + oc), ddx = ldg(P.ddx, mo + oc), ddx_n = ldg(P.ddx, mo + oc + 1);
-  const T rdx = ldg(P.rdx, mo + oc), rdy = ldg(P.rdy, mo + oc), af = ldg(P.af, fo + oc);
-  const T up = ldg(a.up, fo + oc), vp = ldg(a.vp, fo + oc);
+  __syncthreads();
+  if (q0 >= crows * ccols) return;  // whole groups: 4 divides 64
+  const T wn = T(1.0) - a.beta, wo = a.beta;
+
+  // face kl from its 6 points: normal velocity (contravariant), tangential
+  // velocity (covariant) and h
+  const int e0 = iface ? (fr - 3 - ra) * rcols + lj : li * rcols + (fc - 3 - ca);
+  const int es = iface ? rcols : 1;
+  T wf[6], tf[6], hf[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const int e = e0 + m * es;
+    const T u = s[0][e], v = s[1][e], sina = s[6][e];
+    const T g00 = s[2][e], g01 = s[3][e];
+    const T g10 = GSYM ? g01 : s[4][e], g11 = GSYM ? g00 : s[5][e];
+    wf[m] = iface ? (g00 * u + g01 * v) * sina : (g10 * u + g11 * v) * sina;
+    tf[m] = iface ? v : u;
+    hf[m] = s[7][e];
+  }
+  const bool fedge = iface ? (fr == HALO) || (fr == N - HALO) : (fc == HALO) || (fc == N - HALO);
+  T uc, vt, C;
+  iface_b(wf, tf, hf, fedge, fev, fdz, uc, vt, C);
+
+  // pressure gradient along i (even lanes) or j (odd lanes) at (ii, j)
+  const int p0 = pi ? (li - 3) * rcols + lj : li * rcols + lj - 3;
+  const int ps = pi ? rcols : 1;
+  T hb[7];
+#pragma unroll
+  for (int m = 0; m < 7; ++m) hb[m] = wn * s[8][p0 + m * ps] + wo * s[7][p0 + m * ps];
+  const T h0 = c6(hb[0], hb[1], hb[2], hb[3], hb[4], hb[5]);
+  const T h1 = c6(hb[1], hb[2], hb[3], hb[4], hb[5], hb[6]);
+  const T pgf = T(-9.80) * (rpg * (h1 - h0));
+
+  // gather in lane 0 of the group
+  const int base = (static_cast<int>(threadIdx.x) & 63) & ~(kFrameLanes - 1);
+  const T ucx_prev = __shfl(uc, base + 1), vcx_prev = __shfl(vt, base + 1), Cx_prev = __shfl(C, base + 1);
+  const T vcy = __shfl(uc, base + 2), ucy = __shfl(vt, base + 2), Cy = __shfl(C, base + 2);
+  const T vcy_n = __shfl(uc, base + 3), ucy_n = __shfl(vt, base + 3), Cy_n = __shfl(C, base + 3);
+  const T pgfv = __shfl(pgf, base + 1);
+  T cv1[4], cv2[2];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) cv1[m] = __shfl(cin[m], base + 1);
+  cv2[0] = __shfl(cin[0], base + 2);
+  cv2[1] = __shfl(cin[1], base + 2);
+  if (kl != 0) return;
+  const T ucx = uc, vcx = vt, Cx = C, pgfu = pgf;
+  const int ec = li * rcols + lj;
+  const T uc0 = s[0][ec], vc0 = s[1][ec], sc0 = s[6][ec];
+  const T g00c = s[2][ec], g01c = s[3][ec];
+  const T g10c = GSYM ? g01c : s[4][ec], g11c = GSYM ? g00c : s[5][ec];
+  const T vct_c = (g10c * uc0 + g11c * vc0) * sc0;
+  const T uct_c = (g00c * uc0 + g01c * vc0) * sc0;
+  const T cdy_i = cin[0], cdy_prev = cin[1], rda = cin[2], ddx = cin[3];
+  const T ddx_n = cv1[0], rdx = cv1[1], rdy = cv1[2], af = cv1[3];
+  const T up = cv2[0], vp = cv2[1];
   const T zeta = rda * (((ucy * ddx + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
   const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
   const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
   const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
   const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
-  const T uct_c = wu[3];
   const T advx = zeta * vct_c - (ku1 - ku0) * rdx;
   const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
   const T visx = -(rdx * (Cx - Cx_prev));
   const T visy = -(rdy * (Cy_n - Cy));
-  const T hx0 = c6(wb[0], wb[1], wb[2], wb[3], wb[4], wb[5]);
-  const T hx1 = c6(wb[1], wb[2], wb[3], wb[4], wb[5], wb[6]);
-  const T hy0 = c6(sb[0], sb[1], sb[2], sb[3], sb[4], sb[5]);
-  const T hy1 = c6(sb[1], sb[2], sb[3], sb[4], sb[5], sb[6]);
-  const T pgfu = T(-9.80) * (rdx * (hx1 - hx0));
-  const T pgfv = T(-9.80) * (rdy * (hy1 - hy0));
   *at32(a.u_out, fo + oc) = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
   *at32(a.v_out, fo + oc) = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
-}
-
-// Frame cells of a face: rows 3..5 and N-6..N-4, then columns 3..5 and
-// N-6..N-4 of rows 6..N-7 (every interior cell when nx <= 6).
-__host__ __device__ inline int frame_cells(int nx) {
-  const int d = nx > 6 ? nx - 6 : 0;
-  return nx * nx - d * d;
-}
-
-template <typename T, bool GSYM>
-__global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
-  const int nx = a.g.nx, N = a.g.N, per = frame_cells(nx);
-  const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
-  if (q >= 6 * per) return;
-  const int t = q / per;
-  int r = q - t * per, ii, j;
-  if (nx <= 6 || r < 3 * nx) {
-    ii = HALO + r / nx;
-    j = HALO + r % nx;
-  } else if (r < 6 * nx) {
-    r -= 3 * nx;
-    ii = N - 6 + r / nx;
-    j = HALO + r % nx;
-  } else {
-    r -= 6 * nx;
-    ii = 2 * HALO + r / 6;
-    const int k = r % 6;
-    j = k < 3 ? HALO + k : N - 9 + k;
-  }
-  frame_point<T, GSYM>(a, t, ii, j);
 }
 
 // ---------------------------------------------------------------------------
@@ -2214,7 +2267,7 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   if (c->rows_f > 0) {
     // fused height + deep-cell velocity, new-height halo, frame velocity
     const bool sf = A.mf.hp == A.mf.h && A.mf.up == A.mf.u && A.mf.vp == A.mf.v;
-    const dim3 gf((A.mf.nwaves + 3) / 4), gr(blocks_for(6 * static_cast<size_t>(frame_cells(c->nx))));
+    const dim3 gf((A.mf.nwaves + 3) / 4), gr(6 * frame_tiles(c->nx));
     auto lf = [&](auto k) { return launch(c, K_FUSED, k, gf, A.mf); };
     if (c->gsym)
       rc = sf ? lf(k_fused<T, true, true>) : lf(k_fused<T, true, false>);
