@@ -146,10 +146,16 @@ def _summarize(args) -> int:
 
 
 def _find(cs, kernel, counter):
+    """Mean per dispatch of `counter` over every kernel whose name contains
+    `kernel` (all template variants, weighted by their dispatch counts)."""
+    tot = cnt = 0.0
     for (k, c), (n, m) in cs.items():
         if c == counter and kernel in k:
-            return m
-    raise KeyError(f"{counter} for {kernel} not found")
+            tot += n * m
+            cnt += n
+    if not cnt:
+        raise KeyError(f"{counter} for {kernel} not found")
+    return tot / cnt
 
 
 def _traffic(args) -> int:
