@@ -154,6 +154,46 @@ template <typename T>
 __device__ __forceinline__ T celerity(T centred, T dz) {
   return max10<T>(sqrt(T(9.8) * (centred + dz)));
 }
+// 4.9 / c of the LMARS correction (barotr_pure.py:33).
+template <typename T>
+__device__ __forceinline__ T k49_over(T c) {
+  return T(4.9) / c;
+}
+// Contravariant component (ga u + gb v) sina (remap.py:35-39).
+template <typename T>
+__device__ __forceinline__ T contra(T ga, T gb, T u, T v, T sina) {
+  return (ga * u + gb * v) * sina;
+}
+
+// The FP32 form is measured against the FP64 path (relative error), not bit
+// for bit, so it evaluates the same sums as fused multiply-adds in the same
+// association and takes the celerity's square root and reciprocal from the
+// hardware (v_sqrt_f32, v_rcp_f32, 1 ulp).  Every FP32 kernel uses these, so
+// the fused and separate forms and batched runs still agree bit for bit.
+template <>
+__device__ __forceinline__ float e5<float>(float a, float b, float c, float d, float e) {
+  return fmaf(PC<float>::EP23, e, fmaf(PC<float>::EP13, d, fmaf(PC<float>::EP03, c, fmaf(PC<float>::EM13, b, PC<float>::EM23 * a))));
+}
+template <>
+__device__ __forceinline__ float w5<float>(float a, float b, float c, float d, float e) {
+  return fmaf(PC<float>::EM23, e, fmaf(PC<float>::EM13, d, fmaf(PC<float>::EP03, c, fmaf(PC<float>::EP13, b, PC<float>::EP23 * a))));
+}
+template <>
+__device__ __forceinline__ float c6<float>(float m3, float m2, float m1, float p0, float p1, float p2) {
+  return fmaf(PC<float>::M33, m3 + p2, fmaf(PC<float>::M23, m2 + p1, PC<float>::M13 * (m1 + p0)));
+}
+template <>
+__device__ __forceinline__ float celerity<float>(float centred, float dz) {
+  return max10<float>(__builtin_amdgcn_sqrtf(9.8f * (centred + dz)));
+}
+template <>
+__device__ __forceinline__ float k49_over<float>(float c) {
+  return 4.9f * __builtin_amdgcn_rcpf(c);
+}
+template <>
+__device__ __forceinline__ float contra<float>(float ga, float gb, float u, float v, float sina) {
+  return fmaf(ga, u, gb * v) * sina;
+}
 
 // ---------------------------------------------------------------------------
 // context-owned const planes (each (6, N, N))
@@ -187,11 +227,11 @@ __device__ __forceinline__ T dzy(const Planes<T>& P, size_t o) {
 }
 template <typename T>
 __device__ __forceinline__ T uct_at(const Planes<T>& P, const T* u, const T* v, size_t o) {
-  return (P.g00[o] * u[o] + P.g01[o] * v[o]) * P.sina[o];
+  return contra(P.g00[o], P.g01[o], u[o], v[o], P.sina[o]);
 }
 template <typename T>
 __device__ __forceinline__ T vct_at(const Planes<T>& P, const T* u, const T* v, size_t o) {
-  return (P.g10[o] * u[o] + P.g11[o] * v[o]) * P.sina[o];
+  return contra(P.g10[o], P.g11[o], u[o], v[o], P.sina[o]);
 }
 
 // ---------------------------------------------------------------------------
@@ -256,7 +296,7 @@ __device__ __forceinline__ void edge_body(const EdgeArgs<T>& a, int q) {
     const T he = e5(s6[0], s6[1], s6[2], s6[3], s6[4]);
     const T hw = w5(s6[1], s6[2], s6[3], s6[4], s6[5]);
     const T c = celerity(c6(s6[0], s6[1], s6[2], s6[3], s6[4], s6[5]), dz);
-    const T vc = avg + T(4.9) / c * (he - hw);
+    const T vc = avg + k49_over(c) * (he - hw);
     const T f = dz * vc + (vc > T(0) ? he : hw) * vc;
     const T fo = __shfl_xor(f, kEdgeLanes);
     if (k == 0) a.eflux[oe] = (f + fo) / T(2.0);
@@ -372,9 +412,9 @@ struct CtRaw {
 };
 
 template <typename T>
-__device__ __forceinline__ T ct_u(const CtRaw<T>& r) { return (r.g00 * r.u + r.g01 * r.v) * r.sina; }
+__device__ __forceinline__ T ct_u(const CtRaw<T>& r) { return contra(r.g00, r.g01, r.u, r.v, r.sina); }
 template <typename T>
-__device__ __forceinline__ T ct_v(const CtRaw<T>& r) { return (r.g10 * r.u + r.g11 * r.v) * r.sina; }
+__device__ __forceinline__ T ct_v(const CtRaw<T>& r) { return contra(r.g10, r.g11, r.u, r.v, r.sina); }
 
 // Kernel A: new height on the interior (barotr_pure.py:111-120).
 //
@@ -463,7 +503,7 @@ __device__ __forceinline__ T face_flux2(const T* s, const T* w, bool edge, T upr
   const T inner = T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
   const T upre = edge ? upre_edge : inner;
   const T c = celerity(c6(s[0], s[1], s[2], s[3], s[4], s[5]), dz);
-  const T uc = upre + T(4.9) / c * (he - hw);
+  const T uc = upre + k49_over(c) * (he - hw);
   return dz * uc + (uc > T(0) ? he : hw) * uc;
 }
 
@@ -670,7 +710,7 @@ __device__ __forceinline__ void iface_b(const T* wu, const T* wv, const T* wh, b
   const T he = e5(wh[0], wh[1], wh[2], wh[3], wh[4]);
   const T hw = w5(wh[1], wh[2], wh[3], wh[4], wh[5]);
   const T cx = celerity(c6(wh[0], wh[1], wh[2], wh[3], wh[4], wh[5]), dzx);
-  ucx = upre + T(4.9) / cx * (he - hw);
+  ucx = upre + k49_over(cx) * (he - hw);
   Cx = T(0.5) * cx * (ue - uw);
 }
 
@@ -701,12 +741,12 @@ __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s
   for (int m = 0; m < 5; ++m) {
     BCol<T> r;
     b_fetch_col<T, GSYM>(a, w, w.r0 - 5 + m, r);
-    wu[m] = (r.g00 * r.u + r.g01 * r.v) * r.sina;
+    wu[m] = contra(r.g00, r.g01, r.u, r.v, r.sina);
     wv[m] = r.v;
     wh[m] = r.h;
     wn7[m + 1] = wn * r.hn + wo * r.h;
     if (m >= 2) {
-      wvc[m - 2] = (r.g10 * r.u + r.g11 * r.v) * r.sina;
+      wvc[m - 2] = contra(r.g10, r.g11, r.u, r.v, r.sina);
       wuu[m - 2] = r.u;
     }
   }
@@ -722,11 +762,11 @@ __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s
   {
     const BCol<T> c = pcol;
     b_fetch_col<T, GSYM>(a, w, w.r0 + 1, pcol);
-    wu[5] = (c.g00 * c.u + c.g01 * c.v) * c.sina;
+    wu[5] = contra(c.g00, c.g01, c.u, c.v, c.sina);
     wv[5] = c.v;
     wh[5] = c.h;
     wn7[6] = wn * c.hn + wo * c.h;
-    wvc[3] = (c.g10 * c.u + c.g11 * c.v) * c.sina;
+    wvc[3] = contra(c.g10, c.g11, c.u, c.v, c.sina);
     wuu[3] = c.u;
     const BFace<T> f = pface;
     b_fetch_face(a, w, w.r0 + 1, pface);
@@ -755,11 +795,11 @@ __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s
     {
       const BCol<T> c = pcol;
       b_fetch_col<T, GSYM>(a, w, i + 1, pcol);
-      wu[5] = (c.g00 * c.u + c.g01 * c.v) * c.sina;
+      wu[5] = contra(c.g00, c.g01, c.u, c.v, c.sina);
       wv[5] = c.v;
       wh[5] = c.h;
       wn7[6] = wn * c.hn + wo * c.h;
-      wvc[3] = (c.g10 * c.u + c.g11 * c.v) * c.sina;
+      wvc[3] = contra(c.g10, c.g11, c.u, c.v, c.sina);
       wuu[3] = c.u;
     }
     T ucx, vcx, Cx, cdy_i;
@@ -776,7 +816,7 @@ __device__ __forceinline__ void march_b_body(const MarchArgs<T>& a, int L, T (*s
       bu[lane + 3] = wuu[0];
       bh[lane + 3] = wh[2];
       bn[lane + 3] = wn7[3];
-      bvct[hslot] = (hh.g10 * hh.u + hh.g11 * hh.v) * hh.sina;
+      bvct[hslot] = contra(hh.g10, hh.g11, hh.u, hh.v, hh.sina);
       bu[hslot] = hh.u;
       bh[hslot] = hh.h;
       bn[hslot] = wn * hh.hn + wo * hh.h;
@@ -932,7 +972,7 @@ __device__ __forceinline__ FaceQ<T> face_q(const T* s, const T* w, const T* t, b
   const T upre = edge ? upre_edge : inner;
   const T c = celerity(c6(s[0], s[1], s[2], s[3], s[4], s[5]), dz);
   FaceQ<T> q;
-  q.uc = upre + T(4.9) / c * (he - hw);
+  q.uc = upre + k49_over(c) * (he - hw);
   q.F = dz * q.uc + (q.uc > T(0) ? he : hw) * q.uc;
   const T te = e5(t[0], t[1], t[2], t[3], t[4]);
   const T tw = w5(t[1], t[2], t[3], t[4], t[5]);
@@ -1341,7 +1381,7 @@ __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
     const T u = s[0][e], v = s[1][e], sina = s[6][e];
     const T g00 = s[2][e], g01 = s[3][e];
     const T g10 = GSYM ? g01 : s[4][e], g11 = GSYM ? g00 : s[5][e];
-    wf[m] = iface ? (g00 * u + g01 * v) * sina : (g10 * u + g11 * v) * sina;
+    wf[m] = iface ? contra(g00, g01, u, v, sina) : contra(g10, g11, u, v, sina);
     tf[m] = iface ? v : u;
     hf[m] = s[7][e];
   }
@@ -1376,8 +1416,8 @@ __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
   const T uc0 = s[0][ec], vc0 = s[1][ec], sc0 = s[6][ec];
   const T g00c = s[2][ec], g01c = s[3][ec];
   const T g10c = GSYM ? g01c : s[4][ec], g11c = GSYM ? g00c : s[5][ec];
-  const T vct_c = (g10c * uc0 + g11c * vc0) * sc0;
-  const T uct_c = (g00c * uc0 + g01c * vc0) * sc0;
+  const T vct_c = contra(g10c, g11c, uc0, vc0, sc0);
+  const T uct_c = contra(g00c, g01c, uc0, vc0, sc0);
   const T cdy_i = cin[0], cdy_prev = cin[1], rda = cin[2], ddx = cin[3];
   const T ddx_n = cv1[0], rdx = cv1[1], rdy = cv1[2], af = cv1[3];
   const T up = cv2[0], vp = cv2[1];
@@ -1807,7 +1847,7 @@ __global__ __launch_bounds__(256) void k_diag2(DiagArgs<T> a) {
                       w5(U(i - 2, j), U(i - 1, j), U(i, j), U(i + 1, j), U(i + 2, j)));
     const T he = e5(Hh(i - 3, j), Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j));
     const T hw = w5(Hh(i - 2, j), Hh(i - 1, j), Hh(i, j), Hh(i + 1, j), Hh(i + 2, j));
-    ucx = pre + T(4.9) / a.cx[q] * (he - hw);
+    ucx = pre + k49_over(a.cx[q]) * (he - hw);
     const size_t b = f0 + j;
     auto Vv = [&](int ii) { return a.v[b + static_cast<size_t>(ii) * N]; };
     vcx = pre > T(0) ? e5(Vv(i - 3), Vv(i - 2), Vv(i - 1), Vv(i), Vv(i + 1))
@@ -1822,7 +1862,7 @@ __global__ __launch_bounds__(256) void k_diag2(DiagArgs<T> a) {
                       w5(V(i, j - 2), V(i, j - 1), V(i, j), V(i, j + 1), V(i, j + 2)));
     const T hn = e5(Hh(i, j - 3), Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1));
     const T hs = w5(Hh(i, j - 2), Hh(i, j - 1), Hh(i, j), Hh(i, j + 1), Hh(i, j + 2));
-    vcy = pre + T(4.9) / a.cy[q] * (hn - hs);
+    vcy = pre + k49_over(a.cy[q]) * (hn - hs);
     const size_t b = f0 + static_cast<size_t>(i) * N;
     auto Uu = [&](int jj) { return a.u[b + jj]; };
     ucy = pre > T(0) ? e5(Uu(j - 3), Uu(j - 2), Uu(j - 1), Uu(j), Uu(j + 1))
