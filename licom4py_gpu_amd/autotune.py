@@ -104,10 +104,11 @@ def tune(ctx, core6: Sequence, dt: float, substeps: int = 2, reps: int = 2,
 
 def tune_batch(batch, members, candidates: Iterable[int] = CANDIDATES, substeps: int = 4,
                reps: int = 2) -> dict:
-    """Pick one band height for every member of a batched run by timing the
-    whole batch (short bands help a resolution alone, but in a batch the
-    launches are full anyway and short bands only add warm-up rows).
-    Applies and returns it; the members' own substep counts are restored."""
+    """Pick one band height, and the fused or the separate stage kernels, for
+    every member of a batched run by timing the whole batch (short bands help
+    a resolution alone, but in a batch the launches are full anyway and short
+    bands only add warm-up rows).  Applies and returns it; the members' own
+    substep counts are restored."""
     import torch
 
     saved = [m.nbb for m in members]
@@ -116,22 +117,24 @@ def tune_batch(batch, members, candidates: Iterable[int] = CANDIDATES, substeps:
         for m in members:
             m.nbb = substeps
         for r in candidates:
-            for m in members:
-                m.ctx.set_tuning(r, r)
-            batch.run(members, diagnostics=False)
-            torch.cuda.synchronize()
-            best = float("inf")
-            for _ in range(reps):
-                t0 = time.perf_counter()
+            for fused in (False, True):
+                for m in members:
+                    m.ctx.set_tuning(r, r, r if fused else 0)
                 batch.run(members, diagnostics=False)
                 torch.cuda.synchronize()
-                best = min(best, time.perf_counter() - t0)
-            times[r] = best
+                best = float("inf")
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    batch.run(members, diagnostics=False)
+                    torch.cuda.synchronize()
+                    best = min(best, time.perf_counter() - t0)
+                times[(r, fused)] = best
     finally:
         for m, n in zip(members, saved):
             m.nbb = n
-    r = min(times, key=times.get)
+    r, fused = min(times, key=times.get)
     for m in members:
-        m.ctx.set_tuning(r, r)
-    return {"rows": r, "table_ms": {str(k): v * 1e3 for k, v in times.items()}}
+        m.ctx.set_tuning(r, r, r if fused else 0)
+    return {"rows": r, "fused": fused,
+            "table_ms": {f"{k}{'f' if f else 's'}": v * 1e3 for (k, f), v in times.items()}}
 

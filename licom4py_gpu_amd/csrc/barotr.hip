@@ -1301,13 +1301,12 @@ __host__ __device__ inline int frame_tiles(int nx) {
 }
 
 template <typename T, bool GSYM>
-__global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
-  __shared__ T s[9][kTilePts];  // u v g00 g01 g10 g11 sina h hn
+__device__ __forceinline__ void frame_body(const MarchArgs<T>& a, int blk, T (*s)[kTilePts]) {
   const int nx = a.g.nx, N = a.g.N;
   const int nr = (nx + kTile - 1) / kTile, ns = (nx - 6 + kTile - 1) / kTile;
   const int per = 2 * nr + 2 * ns;
-  const int t = static_cast<int>(blockIdx.x) / per;
-  int k = static_cast<int>(blockIdx.x) - t * per, cr0, crows, cc0, ccols;
+  const int t = blk / per;
+  int k = blk - t * per, cr0, crows, cc0, ccols;
   if (k < 2 * nr) {
     const int g = k / nr, c = k - g * nr;
     cr0 = g ? N - 2 * HALO : HALO;
@@ -1432,6 +1431,12 @@ __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
   const T visy = -(rdy * (Cy_n - Cy));
   *at32(a.u_out, fo + oc) = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
   *at32(a.v_out, fo + oc) = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
+  __shared__ T s[9][kTilePts];  // u v g00 g01 g10 g11 sina h hn
+  frame_body<T, GSYM>(a, static_cast<int>(blockIdx.x), s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1645,7 +1650,7 @@ struct StageArgs {
 // its member in a block -> member table and reads the member's arguments from
 // a device table of StageArgs.
 // ---------------------------------------------------------------------------
-enum { BK_EDGE = 0, BK_MARCH_A = 1, BK_HALO = 2, BK_MARCH_B = 3, BK_CLASSES = 4 };
+enum { BK_EDGE = 0, BK_MARCH_A = 1, BK_HALO = 2, BK_MARCH_B = 3, BK_FUSED = 4, BK_FRAME = 5, BK_CLASSES = 6 };
 
 template <typename T>
 struct BatchLaunch {
@@ -1697,6 +1702,24 @@ __global__ __launch_bounds__(256) void k_march_b_batch(BatchLaunch<T> b) {
   const int m = b.blk2m[L];
   const MarchArgs<T> a = batch_entry(b, m).mb;
   march_b_body<T, GSYM, SAMEP>(a, L - b.first[m], s_vct, s_u, s_h, s_n);
+}
+
+template <typename T, bool GSYM, bool SAMEP>
+__global__ __launch_bounds__(256) void k_fused_batch(BatchLaunch<T> b) {
+  __shared__ FRing<T> rings[4];
+  const int L = xcd_block();
+  const int m = b.blk2m[L];
+  const MarchArgs<T> a = batch_entry(b, m).mf;
+  fused_body<T, GSYM, SAMEP>(a, L - b.first[m], rings);
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_frame_batch(BatchLaunch<T> b) {
+  __shared__ T s[9][kTilePts];
+  const int blk = static_cast<int>(blockIdx.x);
+  const int m = b.blk2m[blk];
+  const MarchArgs<T> a = batch_entry(b, m).mf;
+  frame_body<T, GSYM>(a, blk - b.first[m], s);
 }
 
 template <typename T>
@@ -2676,6 +2699,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
   std::vector<const void*> outs;
   std::vector<const void*> ins;
   bool gsym = true;
+  bool fused = true;  // the fused stage kernel when every member's context uses it
   for (int i = 0; i < count; ++i) {
     const sw_member& m = mem[i];
     if (!m.ctx) return fail(SW_E_ARG, "member " + std::to_string(i) + ": ctx is null");
@@ -2692,6 +2716,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
       if (m.out[k]) outs.push_back(m.out[k]);
     for (int k = 0; k < 6; ++k) ins.push_back(m.in[k]);
     gsym = gsym && m.ctx->gsym;
+    fused = fused && m.ctx->rows_f > 0;
   }
   {
     std::vector<const sw_ctx*> ctxs;
@@ -2775,7 +2800,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     const StageArgs<T>& A = tab[q];
     const int blocks[BK_CLASSES] = {static_cast<int>(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))),
                                     (A.ma.nwaves + 3) / 4, static_cast<int>(blocks_for(c->plan_n)),
-                                    (A.mb.nwaves + 3) / 4};
+                                    (A.mb.nwaves + 3) / 4, (A.mf.nwaves + 3) / 4, 6 * frame_tiles(c->nx)};
     for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
   }
   // byte layout of the device copy
@@ -2851,6 +2876,25 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
       // input aliases them
       const bool samep = st == 0 && (k > 0 || in_alias);
       const dim3 ga(first[BK_MARCH_A][active]), gb(first[BK_MARCH_B][active]);
+      if (fused) {
+        const dim3 gf(first[BK_FUSED][active]), gr(first[BK_FRAME][active]);
+        if (gsym && samep)
+          hipLaunchKernelGGL((k_fused_batch<T, true, true>), gf, dim3(256), 0, s, L[BK_FUSED]);
+        else if (gsym)
+          hipLaunchKernelGGL((k_fused_batch<T, true, false>), gf, dim3(256), 0, s, L[BK_FUSED]);
+        else if (samep)
+          hipLaunchKernelGGL((k_fused_batch<T, false, true>), gf, dim3(256), 0, s, L[BK_FUSED]);
+        else
+          hipLaunchKernelGGL((k_fused_batch<T, false, false>), gf, dim3(256), 0, s, L[BK_FUSED]);
+        hipLaunchKernelGGL(k_halo_scalar_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, s, L[BK_HALO]);
+        if (gsym)
+          hipLaunchKernelGGL((k_frame_batch<T, true>), gr, dim3(256), 0, s, L[BK_FRAME]);
+        else
+          hipLaunchKernelGGL((k_frame_batch<T, false>), gr, dim3(256), 0, s, L[BK_FRAME]);
+        hipLaunchKernelGGL(k_halo_vector_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, s, L[BK_HALO]);
+        SW_HIP(hipGetLastError());
+        continue;
+      }
       if (gsym && samep)
         hipLaunchKernelGGL((k_march_a_batch<T, true, true>), ga, dim3(256), 0, s, L[BK_MARCH_A]);
       else if (gsym)
@@ -2881,7 +2925,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     int rc = substep(0, K, stream);
     if (rc) return rc;
     std::vector<long long> sig = {reinterpret_cast<long long>(b->dev), static_cast<long long>(o_tab),
-                                  static_cast<long long>(o_nbb), K, rk, gsym ? 1 : 0,
+                                  static_cast<long long>(o_nbb), K, rk, gsym ? 1 : 0, fused ? 1 : 0,
                                   static_cast<long long>(sizeof(T))};
     for (int k = 0; k < BK_CLASSES; ++k) {
       sig.push_back(static_cast<long long>(o_first[k]));

@@ -116,3 +116,19 @@ def test_batch_rejects_shared_context_and_mixed_dtypes():
     m32 = _members(torch, torch.float32, [(9, 2, 10.0)])
     with pytest.raises(RuntimeError, match="dtype"):
         ensemble.Batch().run(m64 + m32)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_separate_kernels_match_alone(dtype):
+    """Members whose contexts select the separate height / velocity kernels
+    (rows_f = 0) are batched with them, and still equal the fused path alone."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    members = _members(torch, tdt, SPEC, seed=11)
+    for m in members:
+        m.ctx.set_tuning(30, 30, 0)
+    ensemble.Batch().run(members)
+    for m in members:
+        m.ctx.set_tuning(30, 30, 32)  # alone: the fused kernel
+    _check_alone(torch, members)
