@@ -116,9 +116,10 @@ int sw_routing_table(int32_t out[6 * 4 * 3]);
  * ms[k] / counts[k] = total milliseconds / launches of kernel class k:
  * 0 edge lines, 1 height stage kernel, 2 height halo fill, 3 velocity stage
  * kernel, 4 velocity halo fill, 5 diagnostics, 6 fused stage kernel (height
- * and deep-cell velocity), 7 frame velocity.  Classes 1 and 3 run when fusion
- * is off (sw_set_tuning rows_f = 0), 6 and 7 when it is on.  A measurement
- * helper with no counterpart in the reference. */
+ * and deep-cell velocity), 7 frame velocity with the height halo fill.
+ * Classes 1, 2 and 3 run when fusion is off (sw_set_tuning rows_f = 0), 6 and
+ * 7 when it is on.  A measurement helper with no counterpart in the
+ * reference. */
 #define SW_N_KERNEL_CLASSES 8
 int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
                double dtb, int reps, double ms[SW_N_KERNEL_CLASSES], int counts[SW_N_KERNEL_CLASSES]);

@@ -1123,6 +1123,18 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       upl = ldg(a.up, w.fo + om);
       vpl = ldg(a.vp, w.fo + om);
     }
+    // row ia's buffers (written three steps ago) and the FIFO entry of row
+    // ib, read before this step's LDS writes so their latency overlaps the
+    // i-face arithmetic
+    T sv[6], su[6], sh[6], tq0[kFTerms];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      sv[m] = rg.v[sa][lane + m];
+      su[m] = rg.u[sa][lane + m];
+      sh[m] = rg.h[sa][lane + m];
+    }
+#pragma unroll
+    for (int k = 0; k < kFTerms; ++k) tq0[k] = rg.q[sq][k][lane];
     const APre<T> cur = pre;
     fetch(i + STEP, pre);
     cu[NEW] = ct_u(cur.cr);
@@ -1135,22 +1147,10 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     const FaceQ<T> qi = face_q(hh, cu, cv, iedge, cur.evi, cur.dzx);
     const T F = (a.last && iedge) ? cur.efi : qi.F;
 
-    // j-face at column j, row ia (its row buffers were written three steps ago)
-    T vct_c, ucov_c;
-    FaceQ<T> qj;
+    // j-face at column j, row ia
+    const T vct_c = sv[3], ucov_c = su[3];
     const bool jedge = (j == HALO) || (j == N - HALO);
-    {
-      T sv[6], su[6], sh[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        sv[m] = rg.v[sa][lane + m];
-        su[m] = rg.u[sa][lane + m];
-        sh[m] = rg.h[sa][lane + m];
-      }
-      vct_c = sv[3];
-      ucov_c = su[3];
-      qj = face_q(sh, sv, su, jedge, cur.evj, cur.dzy);
-    }
+    const FaceQ<T> qj = face_q(sh, sv, su, jedge, cur.evj, cur.dzy);
     const T G = (a.last && jedge) ? cur.efj : qj.F;
 
     // the cell's bottom (row ia+1) and top (row ia) i-faces
@@ -1218,7 +1218,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     {
       T tq[kFTerms];
 #pragma unroll
-      for (int k = 0; k < kFTerms; ++k) tq[k] = rg.q[sq][k][lane];
+      for (int k = 0; k < kFTerms; ++k) tq[k] = tq0[k];
       const T hx0 = c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]);
       const T hx1 = c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
       const T pgfu = T(-9.80) * (tq[4] * (hx1 - hx0));
@@ -1283,161 +1283,6 @@ __device__ __forceinline__ T f_evj(const MarchArgs<T>& a, int t, int ii, int j) 
   return ldg(a.evel, e ? (t * 4 + (j == HALO ? DW : DE)) * a.g.nx + (ii - HALO) : 0);
 }
 
-// Frame tiles of a face: the two groups of frame rows (3..5, N-6..N-4) in
-// chunks of kTile columns, then the two groups of frame columns (3..5,
-// N-6..N-4) of rows 6..N-7 in chunks of kTile rows (nx >= 7).  A block loads
-// the points its cells' stencils read (the cells and 3 around them) into LDS
-// with row-contiguous loads.  Then a group of 4 lanes takes each cell: lane k
-// evaluates face k of the cell (0 bottom i-face, 1 top i-face, 2 the j-face
-// at j, 3 the j-face at j+1; kernel B's expressions) and even / odd lanes the
-// pressure gradient along i / j, and lane 0 gathers them and completes the
-// update in the reference's order of operations.
-constexpr int kTile = 21;                    // cells along a tile's long side
-constexpr int kTilePts = (kTile + 6) * 9;    // points of a tile's region
-constexpr int kFrameLanes = 4;
-
-__host__ __device__ inline int frame_tiles(int nx) {
-  return 2 * ((nx + kTile - 1) / kTile) + 2 * ((nx - 6 + kTile - 1) / kTile);
-}
-
-template <typename T, bool GSYM>
-__device__ __forceinline__ void frame_body(const MarchArgs<T>& a, int blk, T (*s)[kTilePts]) {
-  const int nx = a.g.nx, N = a.g.N;
-  const int nr = (nx + kTile - 1) / kTile, ns = (nx - 6 + kTile - 1) / kTile;
-  const int per = 2 * nr + 2 * ns;
-  const int t = blk / per;
-  int k = blk - t * per, cr0, crows, cc0, ccols;
-  if (k < 2 * nr) {
-    const int g = k / nr, c = k - g * nr;
-    cr0 = g ? N - 2 * HALO : HALO;
-    crows = 3;
-    cc0 = HALO + kTile * c;
-    ccols = min(kTile, nx - kTile * c);
-  } else {
-    k -= 2 * nr;
-    const int g = k / ns, c = k - g * ns;
-    cr0 = 2 * HALO + kTile * c;
-    crows = min(kTile, nx - 6 - kTile * c);
-    cc0 = g ? N - 2 * HALO : HALO;
-    ccols = 3;
-  }
-  const int ra = cr0 - 3, ca = cc0 - 3, rcols = ccols + 6, npts = (crows + 6) * rcols;
-  const Planes<T>& P = a.P;
-  const uint32_t NS = static_cast<uint32_t>(N);
-  const uint32_t fo = static_cast<uint32_t>(t) * NS * NS, mo = a.shared ? 0 : fo;
-  // the cell of this lane's group (the last cell for lanes past the tile's
-  // cells, which only help fill the tile)
-  const int q0 = static_cast<int>(threadIdx.x) / kFrameLanes, kl = static_cast<int>(threadIdx.x) % kFrameLanes;
-  const int q = min(q0, crows * ccols - 1);
-  const int ii = cr0 + q / ccols, j = cc0 + q % ccols;
-  const int li = ii - ra, lj = j - ca;
-  const bool iface = kl < 2;
-  const int fr = kl == 0 ? ii + 1 : ii, fc = kl == 3 ? j + 1 : j;
-  const bool pi = (kl & 1) == 0;
-  const uint32_t oc = static_cast<uint32_t>(ii) * NS + static_cast<uint32_t>(j);
-  // global inputs besides the tile, issued before the tile loads
-  const T fev = iface ? f_evi(a, t, fr, j) : f_evj(a, t, ii, fc);
-  const uint32_t fd = static_cast<uint32_t>(fr) * NS + static_cast<uint32_t>(fc);
-  const T fdz = iface ? ldg(P.dzx, P.dz_uniform ? 0 : fo + fd) : ldg(P.dzy, P.dz_uniform ? 0 : fo + fd);
-  const T rpg = pi ? ldg(P.rdx, mo + oc) : ldg(P.rdy, mo + oc);
-  // cell inputs of lane 0's update (lane kl loads the kl-th group)
-  const uint32_t on = oc + NS;
-  T cin[4];
-  if (kl == 0) {
-    cin[0] = ldg(P.cdy, mo + on); cin[1] = ldg(P.cdy, mo + oc); cin[2] = ldg(P.rda, mo + oc); cin[3] = ldg(P.ddx, mo + oc);
-  } else if (kl == 1) {
-    cin[0] = ldg(P.ddx, mo + oc + 1); cin[1] = ldg(P.rdx, mo + oc); cin[2] = ldg(P.rdy, mo + oc); cin[3] = ldg(P.af, fo + oc);
-  } else {
-    cin[0] = ldg(a.up, fo + oc); cin[1] = ldg(a.vp, fo + oc); cin[2] = T(0); cin[3] = T(0);
-  }
-  for (int e = static_cast<int>(threadIdx.x); e < npts; e += 256) {
-    const int rr = e / rcols, cc = e - rr * rcols;
-    const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
-    s[0][e] = ldg(a.u, o);
-    s[1][e] = ldg(a.v, o);
-    s[2][e] = ldg(P.g00, o);
-    s[3][e] = ldg(P.g01, o);
-    if (!GSYM) {
-      s[4][e] = ldg(P.g10, o);
-      s[5][e] = ldg(P.g11, o);
-    }
-    s[6][e] = ldg(P.sina, o);
-    s[7][e] = ldg(a.h, o);
-    s[8][e] = ldg(a.hn, o);
-  }
-  __syncthreads();
-  if (q0 >= crows * ccols) return;  // whole groups: 4 divides 64
-  const T wn = T(1.0) - a.beta, wo = a.beta;
-
-  // face kl from its 6 points: normal velocity (contravariant), tangential
-  // velocity (covariant) and h
-  const int e0 = iface ? (fr - 3 - ra) * rcols + lj : li * rcols + (fc - 3 - ca);
-  const int es = iface ? rcols : 1;
-  T wf[6], tf[6], hf[6];
-#pragma unroll
-  for (int m = 0; m < 6; ++m) {
-    const int e = e0 + m * es;
-    const T u = s[0][e], v = s[1][e], sina = s[6][e];
-    const T g00 = s[2][e], g01 = s[3][e];
-    const T g10 = GSYM ? g01 : s[4][e], g11 = GSYM ? g00 : s[5][e];
-    wf[m] = iface ? contra(g00, g01, u, v, sina) : contra(g10, g11, u, v, sina);
-    tf[m] = iface ? v : u;
-    hf[m] = s[7][e];
-  }
-  const bool fedge = iface ? (fr == HALO) || (fr == N - HALO) : (fc == HALO) || (fc == N - HALO);
-  T uc, vt, C;
-  iface_b(wf, tf, hf, fedge, fev, fdz, uc, vt, C);
-
-  // pressure gradient along i (even lanes) or j (odd lanes) at (ii, j)
-  const int p0 = pi ? (li - 3) * rcols + lj : li * rcols + lj - 3;
-  const int ps = pi ? rcols : 1;
-  T hb[7];
-#pragma unroll
-  for (int m = 0; m < 7; ++m) hb[m] = wn * s[8][p0 + m * ps] + wo * s[7][p0 + m * ps];
-  const T h0 = c6(hb[0], hb[1], hb[2], hb[3], hb[4], hb[5]);
-  const T h1 = c6(hb[1], hb[2], hb[3], hb[4], hb[5], hb[6]);
-  const T pgf = T(-9.80) * (rpg * (h1 - h0));
-
-  // gather in lane 0 of the group
-  const int base = (static_cast<int>(threadIdx.x) & 63) & ~(kFrameLanes - 1);
-  const T ucx_prev = __shfl(uc, base + 1), vcx_prev = __shfl(vt, base + 1), Cx_prev = __shfl(C, base + 1);
-  const T vcy = __shfl(uc, base + 2), ucy = __shfl(vt, base + 2), Cy = __shfl(C, base + 2);
-  const T vcy_n = __shfl(uc, base + 3), ucy_n = __shfl(vt, base + 3), Cy_n = __shfl(C, base + 3);
-  const T pgfv = __shfl(pgf, base + 1);
-  T cv1[4], cv2[2];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) cv1[m] = __shfl(cin[m], base + 1);
-  cv2[0] = __shfl(cin[0], base + 2);
-  cv2[1] = __shfl(cin[1], base + 2);
-  if (kl != 0) return;
-  const T ucx = uc, vcx = vt, Cx = C, pgfu = pgf;
-  const int ec = li * rcols + lj;
-  const T uc0 = s[0][ec], vc0 = s[1][ec], sc0 = s[6][ec];
-  const T g00c = s[2][ec], g01c = s[3][ec];
-  const T g10c = GSYM ? g01c : s[4][ec], g11c = GSYM ? g00c : s[5][ec];
-  const T vct_c = contra(g10c, g11c, uc0, vc0, sc0);
-  const T uct_c = contra(g00c, g01c, uc0, vc0, sc0);
-  const T cdy_i = cin[0], cdy_prev = cin[1], rda = cin[2], ddx = cin[3];
-  const T ddx_n = cv1[0], rdx = cv1[1], rdy = cv1[2], af = cv1[3];
-  const T up = cv2[0], vp = cv2[1];
-  const T zeta = rda * (((ucy * ddx + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
-  const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
-  const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
-  const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
-  const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
-  const T advx = zeta * vct_c - (ku1 - ku0) * rdx;
-  const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
-  const T visx = -(rdx * (Cx - Cx_prev));
-  const T visy = -(rdy * (Cy_n - Cy));
-  *at32(a.u_out, fo + oc) = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
-  *at32(a.v_out, fo + oc) = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
-}
-
-template <typename T, bool GSYM>
-__global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a) {
-  __shared__ T s[9][kTilePts];  // u v g00 g01 g10 g11 sina h hn
-  frame_body<T, GSYM>(a, static_cast<int>(blockIdx.x), s);
-}
 
 // ---------------------------------------------------------------------------
 // duo-grid halo fill (update_domain pass 1 + cube_rmp, closed index form)
@@ -1636,6 +1481,188 @@ __global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
   halo_vector_body(a, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
+// Frame tiles of a face: the two groups of frame rows (3..5, N-6..N-4) in
+// chunks of kTile columns, then the two groups of frame columns (3..5,
+// N-6..N-4) of rows 6..N-7 in chunks of kTile rows (nx >= 7).  A block loads
+// the points its cells' stencils read (the cells and 3 around them) into LDS
+// with row-contiguous loads.  Then a group of 4 lanes takes each cell: lane k
+// evaluates face k of the cell (0 bottom i-face, 1 top i-face, 2 the j-face
+// at j, 3 the j-face at j+1; kernel B's expressions) and even / odd lanes the
+// pressure gradient along i / j, and lane 0 gathers them and completes the
+// update in the reference's order of operations.
+constexpr int kTile = 21;                    // cells along a tile's long side
+constexpr int kTilePts = (kTile + 6) * 9;    // points of a tile's region
+constexpr int kFrameLanes = 4;
+
+__host__ __device__ inline int frame_tiles(int nx) {
+  return 2 * ((nx + kTile - 1) / kTile) + 2 * ((nx - 6 + kTile - 1) / kTile);
+}
+
+template <typename T, bool GSYM>
+__device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan<T>& plan, int blk,
+                                           T (*s)[kTilePts]) {
+  const int nx = a.g.nx, N = a.g.N;
+  const int nr = (nx + kTile - 1) / kTile, ns = (nx - 6 + kTile - 1) / kTile;
+  const int per = 2 * nr + 2 * ns;
+  const int t = blk / per;
+  int k = blk - t * per, cr0, crows, cc0, ccols;
+  if (k < 2 * nr) {
+    const int g = k / nr, c = k - g * nr;
+    cr0 = g ? N - 2 * HALO : HALO;
+    crows = 3;
+    cc0 = HALO + kTile * c;
+    ccols = min(kTile, nx - kTile * c);
+  } else {
+    k -= 2 * nr;
+    const int g = k / ns, c = k - g * ns;
+    cr0 = 2 * HALO + kTile * c;
+    crows = min(kTile, nx - 6 - kTile * c);
+    cc0 = g ? N - 2 * HALO : HALO;
+    ccols = 3;
+  }
+  const int ra = cr0 - 3, ca = cc0 - 3, rcols = ccols + 6, npts = (crows + 6) * rcols;
+  const Planes<T>& P = a.P;
+  const uint32_t NS = static_cast<uint32_t>(N);
+  const uint32_t fo = static_cast<uint32_t>(t) * NS * NS, mo = a.shared ? 0 : fo;
+  // the cell of this lane's group (the last cell for lanes past the tile's
+  // cells, which only help fill the tile)
+  const int q0 = static_cast<int>(threadIdx.x) / kFrameLanes, kl = static_cast<int>(threadIdx.x) % kFrameLanes;
+  const int q = min(q0, crows * ccols - 1);
+  const int ii = cr0 + q / ccols, j = cc0 + q % ccols;
+  const int li = ii - ra, lj = j - ca;
+  const bool iface = kl < 2;
+  const int fr = kl == 0 ? ii + 1 : ii, fc = kl == 3 ? j + 1 : j;
+  const bool pi = (kl & 1) == 0;
+  const uint32_t oc = static_cast<uint32_t>(ii) * NS + static_cast<uint32_t>(j);
+  // global inputs besides the tile, issued before the tile loads
+  const T fev = iface ? f_evi(a, t, fr, j) : f_evj(a, t, ii, fc);
+  const uint32_t fd = static_cast<uint32_t>(fr) * NS + static_cast<uint32_t>(fc);
+  const T fdz = iface ? ldg(P.dzx, P.dz_uniform ? 0 : fo + fd) : ldg(P.dzy, P.dz_uniform ? 0 : fo + fd);
+  const T rpg = pi ? ldg(P.rdx, mo + oc) : ldg(P.rdy, mo + oc);
+  // cell inputs of lane 0's update (lane kl loads the kl-th group)
+  const uint32_t on = oc + NS;
+  T cin[4];
+  if (kl == 0) {
+    cin[0] = ldg(P.cdy, mo + on); cin[1] = ldg(P.cdy, mo + oc); cin[2] = ldg(P.rda, mo + oc); cin[3] = ldg(P.ddx, mo + oc);
+  } else if (kl == 1) {
+    cin[0] = ldg(P.ddx, mo + oc + 1); cin[1] = ldg(P.rdx, mo + oc); cin[2] = ldg(P.rdy, mo + oc); cin[3] = ldg(P.af, fo + oc);
+  } else {
+    cin[0] = ldg(a.up, fo + oc); cin[1] = ldg(a.vp, fo + oc); cin[2] = T(0); cin[3] = T(0);
+  }
+  for (int e = static_cast<int>(threadIdx.x); e < npts; e += 256) {
+    const int rr = e / rcols, cc = e - rr * rcols;
+    const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
+    s[0][e] = ldg(a.u, o);
+    s[1][e] = ldg(a.v, o);
+    s[2][e] = ldg(P.g00, o);
+    s[3][e] = ldg(P.g01, o);
+    if (!GSYM) {
+      s[4][e] = ldg(P.g10, o);
+      s[5][e] = ldg(P.g11, o);
+    }
+    s[6][e] = ldg(P.sina, o);
+    s[7][e] = ldg(a.h, o);
+    // the new height on a strip halo cell is the scalar fill's value, formed
+    // here from the halo plan (its sources are interior cells), so the fill
+    // may run in the same launch; corner cells are never read
+    const int row = ra + rr, col = ca + cc;
+    const bool hrow = row < HALO || row >= N - HALO, hcol = col < HALO || col >= N - HALO;
+    if (hrow != hcol) {
+      const int entry = col >= N - HALO ? 3 * (row - HALO) + (col - (N - HALO))
+                      : col < HALO     ? 3 * nx + 3 * (row - HALO) + (HALO - 1 - col)
+                      : row >= N - HALO ? 6 * nx + (row - (N - HALO)) * nx + (col - HALO)
+                                        : 9 * nx + (HALO - 1 - row) * nx + (col - HALO);
+      const int pq = t * (12 * nx + 36) + entry;
+      s[8][e] = plan.c0[pq] * ldg(a.hn, plan.s0[pq]) + plan.c1[pq] * ldg(a.hn, plan.s1[pq]);
+    } else {
+      s[8][e] = ldg(a.hn, o);
+    }
+  }
+  __syncthreads();
+  if (q0 >= crows * ccols) return;  // whole groups: 4 divides 64
+  const T wn = T(1.0) - a.beta, wo = a.beta;
+
+  // face kl from its 6 points: normal velocity (contravariant), tangential
+  // velocity (covariant) and h
+  const int e0 = iface ? (fr - 3 - ra) * rcols + lj : li * rcols + (fc - 3 - ca);
+  const int es = iface ? rcols : 1;
+  T wf[6], tf[6], hf[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const int e = e0 + m * es;
+    const T u = s[0][e], v = s[1][e], sina = s[6][e];
+    const T g00 = s[2][e], g01 = s[3][e];
+    const T g10 = GSYM ? g01 : s[4][e], g11 = GSYM ? g00 : s[5][e];
+    wf[m] = iface ? contra(g00, g01, u, v, sina) : contra(g10, g11, u, v, sina);
+    tf[m] = iface ? v : u;
+    hf[m] = s[7][e];
+  }
+  const bool fedge = iface ? (fr == HALO) || (fr == N - HALO) : (fc == HALO) || (fc == N - HALO);
+  T uc, vt, C;
+  iface_b(wf, tf, hf, fedge, fev, fdz, uc, vt, C);
+
+  // pressure gradient along i (even lanes) or j (odd lanes) at (ii, j)
+  const int p0 = pi ? (li - 3) * rcols + lj : li * rcols + lj - 3;
+  const int ps = pi ? rcols : 1;
+  T hb[7];
+#pragma unroll
+  for (int m = 0; m < 7; ++m) hb[m] = wn * s[8][p0 + m * ps] + wo * s[7][p0 + m * ps];
+  const T h0 = c6(hb[0], hb[1], hb[2], hb[3], hb[4], hb[5]);
+  const T h1 = c6(hb[1], hb[2], hb[3], hb[4], hb[5], hb[6]);
+  const T pgf = T(-9.80) * (rpg * (h1 - h0));
+
+  // gather in lane 0 of the group
+  const int base = (static_cast<int>(threadIdx.x) & 63) & ~(kFrameLanes - 1);
+  const T ucx_prev = __shfl(uc, base + 1), vcx_prev = __shfl(vt, base + 1), Cx_prev = __shfl(C, base + 1);
+  const T vcy = __shfl(uc, base + 2), ucy = __shfl(vt, base + 2), Cy = __shfl(C, base + 2);
+  const T vcy_n = __shfl(uc, base + 3), ucy_n = __shfl(vt, base + 3), Cy_n = __shfl(C, base + 3);
+  const T pgfv = __shfl(pgf, base + 1);
+  T cv1[4], cv2[2];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) cv1[m] = __shfl(cin[m], base + 1);
+  cv2[0] = __shfl(cin[0], base + 2);
+  cv2[1] = __shfl(cin[1], base + 2);
+  if (kl != 0) return;
+  const T ucx = uc, vcx = vt, Cx = C, pgfu = pgf;
+  const int ec = li * rcols + lj;
+  const T uc0 = s[0][ec], vc0 = s[1][ec], sc0 = s[6][ec];
+  const T g00c = s[2][ec], g01c = s[3][ec];
+  const T g10c = GSYM ? g01c : s[4][ec], g11c = GSYM ? g00c : s[5][ec];
+  const T vct_c = contra(g10c, g11c, uc0, vc0, sc0);
+  const T uct_c = contra(g00c, g01c, uc0, vc0, sc0);
+  const T cdy_i = cin[0], cdy_prev = cin[1], rda = cin[2], ddx = cin[3];
+  const T ddx_n = cv1[0], rdx = cv1[1], rdy = cv1[2], af = cv1[3];
+  const T up = cv2[0], vp = cv2[1];
+  const T zeta = rda * (((ucy * ddx + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
+  const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
+  const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
+  const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
+  const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+  const T advx = zeta * vct_c - (ku1 - ku0) * rdx;
+  const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
+  const T visx = -(rdx * (Cx - Cx_prev));
+  const T visy = -(rdy * (Cy_n - Cy));
+  *at32(a.u_out, fo + oc) = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
+  *at32(a.v_out, fo + oc) = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
+}
+
+// The frame tiles (blocks 0 .. nft-1) and the new height's halo fill (the
+// remaining blocks) in one launch.
+template <typename T, bool GSYM>
+__device__ __forceinline__ void frame_h_body(const MarchArgs<T>& a, const HaloArgs<T>& hh, int nft, int blk,
+                                             T (*s)[kTilePts]) {
+  if (blk < nft)
+    frame_body<T, GSYM>(a, hh.plan, blk, s);
+  else
+    halo_scalar_body(hh, (blk - nft) * 256 + static_cast<int>(threadIdx.x));
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a, HaloArgs<T> hh, int nft) {
+  __shared__ T s[9][kTilePts];  // u v g00 g01 g10 g11 sina h hn
+  frame_h_body<T, GSYM>(a, hh, nft, static_cast<int>(blockIdx.x), s);
+}
+
 // Kernel arguments of one RK stage of one context.
 template <typename T>
 struct StageArgs {
@@ -1718,8 +1745,8 @@ __global__ __launch_bounds__(256) void k_frame_batch(BatchLaunch<T> b) {
   __shared__ T s[9][kTilePts];
   const int blk = static_cast<int>(blockIdx.x);
   const int m = b.blk2m[blk];
-  const MarchArgs<T> a = batch_entry(b, m).mf;
-  frame_body<T, GSYM>(a, blk - b.first[m], s);
+  const StageArgs<T>& e = batch_entry(b, m);
+  frame_h_body<T, GSYM>(e.mf, e.hh, 6 * frame_tiles(e.mf.g.nx), blk - b.first[m], s);
 }
 
 template <typename T>
@@ -2169,10 +2196,10 @@ enum { K_EDGE = 0, K_MARCH_A = 1, K_HALO_H = 2, K_MARCH_B = 3, K_HALO_UV = 4, K_
 // launch carries a start / stop hipEvent on its own dispatch
 // (hipExtLaunchKernelGGL), so the events time the kernel itself with no
 // marker packets between kernels.
-template <typename K, typename A>
-int launch(const sw_ctx* c, int cls, K kernel, dim3 grid, const A& args) {
+template <typename K, typename... A>
+int launch(const sw_ctx* c, int cls, K kernel, dim3 grid, const A&... args) {
   if (!c->prof) {
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, args);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, args...);
     SW_HIP(hipGetLastError());
     return SW_OK;
   }
@@ -2181,7 +2208,7 @@ int launch(const sw_ctx* c, int cls, K kernel, dim3 grid, const A& args) {
     SW_HIP(hipEventCreate(&rec.a));
     SW_HIP(hipEventCreate(&rec.b));
     c->prof->push_back(rec);
-    hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, rec.a, rec.b, 0, args);
+    hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, rec.a, rec.b, 0, args...);
     SW_HIP(hipGetLastError());
   }
   return SW_OK;
@@ -2330,15 +2357,17 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   if (c->rows_f > 0) {
     // fused height + deep-cell velocity, new-height halo, frame velocity
     const bool sf = A.mf.hp == A.mf.h && A.mf.up == A.mf.u && A.mf.vp == A.mf.v;
-    const dim3 gf((A.mf.nwaves + 3) / 4), gr(6 * frame_tiles(c->nx));
+    const int nft = 6 * frame_tiles(c->nx);
+    const dim3 gf((A.mf.nwaves + 3) / 4), gr(nft + static_cast<int>(blocks_for(c->plan_n)));
     auto lf = [&](auto k) { return launch(c, K_FUSED, k, gf, A.mf); };
     if (c->gsym)
       rc = sf ? lf(k_fused<T, true, true>) : lf(k_fused<T, true, false>);
     else
       rc = sf ? lf(k_fused<T, false, true>) : lf(k_fused<T, false, false>);
     if (rc) return rc;
-    if ((rc = launch(c, K_HALO_H, k_halo_scalar<T>, gh, A.hh))) return rc;
-    rc = c->gsym ? launch(c, K_FRAME, k_frame<T, true>, gr, A.mf) : launch(c, K_FRAME, k_frame<T, false>, gr, A.mf);
+    // frame cells and the new height's halo fill in one launch
+    rc = c->gsym ? launch(c, K_FRAME, k_frame<T, true>, gr, A.mf, A.hh, nft)
+                 : launch(c, K_FRAME, k_frame<T, false>, gr, A.mf, A.hh, nft);
     if (rc) return rc;
     return launch(c, K_HALO_UV, k_halo_vector<T>, gh, A.huv);
   }
@@ -2800,7 +2829,8 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     const StageArgs<T>& A = tab[q];
     const int blocks[BK_CLASSES] = {static_cast<int>(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))),
                                     (A.ma.nwaves + 3) / 4, static_cast<int>(blocks_for(c->plan_n)),
-                                    (A.mb.nwaves + 3) / 4, (A.mf.nwaves + 3) / 4, 6 * frame_tiles(c->nx)};
+                                    (A.mb.nwaves + 3) / 4, (A.mf.nwaves + 3) / 4,
+                                    6 * frame_tiles(c->nx) + static_cast<int>(blocks_for(c->plan_n))};
     for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
   }
   // byte layout of the device copy
@@ -2886,7 +2916,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
           hipLaunchKernelGGL((k_fused_batch<T, false, true>), gf, dim3(256), 0, s, L[BK_FUSED]);
         else
           hipLaunchKernelGGL((k_fused_batch<T, false, false>), gf, dim3(256), 0, s, L[BK_FUSED]);
-        hipLaunchKernelGGL(k_halo_scalar_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, s, L[BK_HALO]);
+        // frame cells and the new height's halo fill in one launch
         if (gsym)
           hipLaunchKernelGGL((k_frame_batch<T, true>), gr, dim3(256), 0, s, L[BK_FRAME]);
         else
