@@ -178,7 +178,9 @@ def main() -> int:
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                     "traffic": traffic, "algorithmic_bytes_per_launch": int(alg),
-                    "avg_launch_us": round(avg_s * 1e6, 3)}
+                    "avg_launch_us": round(avg_s * 1e6, 3),
+                    # measured HBM bytes (counters) over the same launch time
+                    "traffic_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None}
     path_ach = value * path_bytes / 1e9
     roofline_path = {"bound": "hbm", "achieved": round(path_ach, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(path_ach / PEAK_HBM_GBS, 4),
