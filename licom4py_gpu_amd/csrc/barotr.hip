@@ -237,6 +237,51 @@ __device__ __forceinline__ T vct_at(const Planes<T>& P, const T* u, const T* v, 
 // ---------------------------------------------------------------------------
 // edge lines
 // ---------------------------------------------------------------------------
+// Halo plan (built by k_plan_build): entry q holds the halo cell's offset,
+// its two interpolation sources and weights and the rotation coefficients a
+// vector fill reads.
+constexpr int kPlanRot = 12;
+
+template <typename T>
+struct HaloPlan {
+  const uint32_t *dst, *s0, *s1;
+  const T *c0, *c1;
+  const T* rot;  // kPlanRot x n: c2l(s0) 00 01 10 11, c2l(s1) 00 01 10 11, l2c(dst) 00 01 10 11
+  int n;
+};
+
+// Plan entry of the strip halo cell (row, col) of face t (not a corner): per
+// face the W/E strips (entries (d, r, layer)), then the N/S strips ((d,
+// layer, r)), then the corners, as k_plan_build numbers them.
+__device__ __forceinline__ int halo_entry(int nx, int N, int t, int row, int col) {
+  const int e = col >= N - HALO ? 3 * (row - HALO) + (col - (N - HALO))
+              : col < HALO      ? 3 * nx + 3 * (row - HALO) + (HALO - 1 - col)
+              : row >= N - HALO ? 6 * nx + (row - (N - HALO)) * nx + (col - HALO)
+                                : 9 * nx + (HALO - 1 - row) * nx + (col - HALO);
+  return t * (12 * nx + 36) + e;
+}
+
+// The vector fill's value of plan entry q (remap to lat-lon at the sources,
+// interpolation, remap back; cube.py:137-153), as k_halo_vector forms it.
+template <typename T>
+__device__ __forceinline__ void halo_uv_value(const HaloPlan<T>& plan, const T* u, const T* v, int q, T& uo, T& vo) {
+  const int n = plan.n;
+  const uint32_t s0 = plan.s0[q], s1 = plan.s1[q];
+  const T c0 = plan.c0[q], c1 = plan.c1[q];
+  T R[kPlanRot];
+#pragma unroll
+  for (int k = 0; k < kPlanRot; ++k) R[k] = plan.rot[static_cast<size_t>(k) * n + q];
+  const T u0 = u[s0], v0 = v[s0], u1 = u[s1], v1 = v[s1];
+  const T ul0 = R[0] * u0 + R[1] * v0;
+  const T vl0 = R[2] * u0 + R[3] * v0;
+  const T ul1 = R[4] * u1 + R[5] * v1;
+  const T vl1 = R[6] * u1 + R[7] * v1;
+  const T p0 = c0 * ul0 + c1 * ul1;
+  const T p1 = c0 * vl0 + c1 * vl1;
+  uo = R[8] * p0 + R[9] * p1;
+  vo = R[10] * p0 + R[11] * p1;
+}
+
 template <typename T>
 struct EdgeArgs {
   Geo g;
@@ -246,6 +291,10 @@ struct EdgeArgs {
   T *evel, *eflux;
   int last;
   int ue[12];  // the 12 cube edges, each once: t * 4 + d with (t, d) < its neighbour
+  // fly: u, v on halo cells are formed from the plan instead of read, so the
+  // pass can share a launch with the vector halo fill of u, v
+  int fly;
+  HaloPlan<T> plan;
 };
 
 // Edge values of both faces meeting at a cube edge (communication.py:485-564
@@ -278,7 +327,15 @@ __device__ __forceinline__ void edge_body(const EdgeArgs<T>& a, int q) {
   const size_t o = ns ? a.g.at(ts, line - 3 + kk, 3 + rs) : a.g.at(ts, 3 + rs, line - 3 + kk);
   const size_t oc = ns ? a.g.at(ts, line, 3 + rs) : a.g.at(ts, 3 + rs, line);
   const Planes<T>& P = a.P;
-  const T val = ns ? uct_at(P, a.u, a.v, o) : vct_at(P, a.u, a.v, o);
+  T val;
+  const int prow = ns ? line - 3 + kk : 3 + rs, pcol = ns ? 3 + rs : line - 3 + kk;
+  if (a.fly && (ns ? (prow < HALO || prow >= N - HALO) : (pcol < HALO || pcol >= N - HALO))) {
+    T uh, vh;
+    halo_uv_value(a.plan, a.u, a.v, halo_entry(nx, N, ts, prow, pcol), uh, vh);
+    val = ns ? contra(P.g00[o], P.g01[o], uh, vh, P.sina[o]) : contra(P.g10[o], P.g11[o], uh, vh, P.sina[o]);
+  } else {
+    val = ns ? uct_at(P, a.u, a.v, o) : vct_at(P, a.u, a.v, o);
+  }
   const int base = (threadIdx.x & 63) & ~(kEdgeLanes - 1);
   T w[6];
 #pragma unroll
@@ -1366,15 +1423,6 @@ __device__ __forceinline__ void halo_cell(int N, int nx, int q, int& i, int& j, 
 // arrays instead of scattered lines of ten (6, N, N) planes.  Entries of the
 // W/E strips run layer fastest: three neighbouring lanes write three adjacent
 // cells of one row.
-constexpr int kPlanRot = 12;
-
-template <typename T>
-struct HaloPlan {
-  const uint32_t *dst, *s0, *s1;
-  const T *c0, *c1;
-  const T* rot;  // kPlanRot x n: c2l(s0) 00 01 10 11, c2l(s1) 00 01 10 11, l2c(dst) 00 01 10 11
-  int n;
-};
 
 template <typename T>
 struct PlanBuildArgs {
@@ -1460,25 +1508,28 @@ template <typename T>
 __device__ __forceinline__ void halo_vector_body(const HaloArgs<T>& a, int q) {
   const int n = a.plan.n;
   if (q >= n) return;
-  const uint32_t d = a.plan.dst[q], s0 = a.plan.s0[q], s1 = a.plan.s1[q];
-  const T c0 = a.plan.c0[q], c1 = a.plan.c1[q];
-  T R[kPlanRot];
-#pragma unroll
-  for (int k = 0; k < kPlanRot; ++k) R[k] = a.plan.rot[static_cast<size_t>(k) * n + q];
-  const T u0 = a.u[s0], v0 = a.v[s0], u1 = a.u[s1], v1 = a.v[s1];
-  const T ul0 = R[0] * u0 + R[1] * v0;
-  const T vl0 = R[2] * u0 + R[3] * v0;
-  const T ul1 = R[4] * u1 + R[5] * v1;
-  const T vl1 = R[6] * u1 + R[7] * v1;
-  const T p0 = c0 * ul0 + c1 * ul1;
-  const T p1 = c0 * vl0 + c1 * vl1;
-  a.u[d] = R[8] * p0 + R[9] * p1;
-  a.v[d] = R[10] * p0 + R[11] * p1;
+  const uint32_t d = a.plan.dst[q];
+  T uo, vo;
+  halo_uv_value(a.plan, a.u, a.v, q, uo, vo);
+  a.u[d] = uo;
+  a.v[d] = vo;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_halo_vector(HaloArgs<T> a) {
   halo_vector_body(a, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// The velocity halo fill of a stage (blocks 0 .. nh-1) and the next stage's
+// edge pass on the same fields (the remaining blocks), which forms the halo
+// values it reads from the plan.
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_uv_edge(HaloArgs<T> huv, EdgeArgs<T> en, int nh) {
+  const int blk = static_cast<int>(blockIdx.x);
+  if (blk < nh)
+    halo_vector_body(huv, blk * 256 + static_cast<int>(threadIdx.x));
+  else
+    edge_body(en, (blk - nh) * 256 + static_cast<int>(threadIdx.x));
 }
 
 // Frame tiles of a face: the two groups of frame rows (3..5, N-6..N-4) in
@@ -1568,11 +1619,7 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
     const int row = ra + rr, col = ca + cc;
     const bool hrow = row < HALO || row >= N - HALO, hcol = col < HALO || col >= N - HALO;
     if (hrow != hcol) {
-      const int entry = col >= N - HALO ? 3 * (row - HALO) + (col - (N - HALO))
-                      : col < HALO     ? 3 * nx + 3 * (row - HALO) + (HALO - 1 - col)
-                      : row >= N - HALO ? 6 * nx + (row - (N - HALO)) * nx + (col - HALO)
-                                        : 9 * nx + (HALO - 1 - row) * nx + (col - HALO);
-      const int pq = t * (12 * nx + 36) + entry;
+      const int pq = halo_entry(nx, N, t, row, col);
       s[8][e] = plan.c0[pq] * ldg(a.hn, plan.s0[pq]) + plan.c1[pq] * ldg(a.hn, plan.s1[pq]);
     } else {
       s[8][e] = ldg(a.hn, o);
@@ -1666,7 +1713,7 @@ __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a, HaloArgs<T> hh, i
 // Kernel arguments of one RK stage of one context.
 template <typename T>
 struct StageArgs {
-  EdgeArgs<T> e;
+  EdgeArgs<T> e, en;  // this stage's edge pass and the next stage's
   MarchArgs<T> ma, mb, mf;  // kernel A, kernel B, fused kernel and frame
   HaloArgs<T> hh, huv;
 };
@@ -1677,7 +1724,8 @@ struct StageArgs {
 // its member in a block -> member table and reads the member's arguments from
 // a device table of StageArgs.
 // ---------------------------------------------------------------------------
-enum { BK_EDGE = 0, BK_MARCH_A = 1, BK_HALO = 2, BK_MARCH_B = 3, BK_FUSED = 4, BK_FRAME = 5, BK_CLASSES = 6 };
+enum { BK_EDGE = 0, BK_MARCH_A = 1, BK_HALO = 2, BK_MARCH_B = 3, BK_FUSED = 4, BK_FRAME = 5, BK_HUVE = 6,
+       BK_CLASSES = 7 };
 
 template <typename T>
 struct BatchLaunch {
@@ -1757,12 +1805,18 @@ __global__ __launch_bounds__(256) void k_halo_scalar_batch(BatchLaunch<T> b) {
   halo_scalar_body(a, (blk - b.first[m]) * 256 + static_cast<int>(threadIdx.x));
 }
 
+// Each member's velocity halo fill and its next stage's edge pass (as
+// k_halo_uv_edge).
 template <typename T>
-__global__ __launch_bounds__(256) void k_halo_vector_batch(BatchLaunch<T> b) {
+__global__ __launch_bounds__(256) void k_halo_uv_edge_batch(BatchLaunch<T> b) {
   const int blk = static_cast<int>(blockIdx.x);
   const int m = b.blk2m[blk];
-  const HaloArgs<T> a = batch_entry(b, m).huv;
-  halo_vector_body(a, (blk - b.first[m]) * 256 + static_cast<int>(threadIdx.x));
+  const StageArgs<T>& e = batch_entry(b, m);
+  const int local = blk - b.first[m], nh = (e.huv.plan.n + 255) / 256;
+  if (local < nh)
+    halo_vector_body(e.huv, local * 256 + static_cast<int>(threadIdx.x));
+  else
+    edge_body(e.en, (local - nh) * 256 + static_cast<int>(threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -2219,8 +2273,13 @@ T* scratch_field(const sw_ctx* c, int k) {  // k in [0, 12)
   return static_cast<T*>(c->scratch) + static_cast<size_t>(k) * c->cells;
 }
 
-// Edge buffers: 0 averaged edge velocity, 1 averaged edge flux, 2 scratch of
-// sw_edge_avg, 3 store sink of the stage kernels (each >= 256 elements).
+// Edge buffers (each >= 256 elements): the averaged edge velocity and flux of
+// RK stage index s at 2 s and 2 s + 1 (s < 3: the velocity halo launch of a
+// stage forms the next stage's while the stage kernels of this one may still
+// be read by the diagnostics), 6 scratch of sw_edge_avg, 7 store sink of the
+// stage kernels.
+constexpr int kEdgeBufs = 8;
+constexpr int kEdgeScratch = 6, kEdgeSink = 7;
 size_t edge_stride(const sw_ctx* c) {
   const size_t n = 24 * static_cast<size_t>(c->nx);
   return n < 256 ? 256 : n;
@@ -2302,15 +2361,21 @@ int edge_pairs(int ue[12]) {
 
 template <typename T>
 int stage_args(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, const T* up,
-               const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last,
-               StageArgs<T>& A) {
+               const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last, int st,
+               int rk, StageArgs<T>& A) {
   const Geo g = geo_of(c);
   const Planes<T> P = planes_of<T>(c);
-  T* evel = edge_buf<T>(c, 0);
-  T* eflux = edge_buf<T>(c, 1);
-  A.e = EdgeArgs<T>{g, route(), P, h, u, v, evel, eflux, last, {}};
+  T* evel = edge_buf<T>(c, 2 * st);
+  T* eflux = edge_buf<T>(c, 2 * st + 1);
+  A.e = EdgeArgs<T>{g, route(), P, h, u, v, evel, eflux, last, {}, 0, {}};
   int rc = edge_pairs(A.e.ue);
   if (rc) return rc;
+  // the next stage's edge pass, on this stage's outputs, shares the launch of
+  // their velocity halo fill
+  const int nst = (st + 1) % rk;
+  A.en = EdgeArgs<T>{g, route(), P, h_out, u_out, v_out, edge_buf<T>(c, 2 * nst), edge_buf<T>(c, 2 * nst + 1),
+                     nst == rk - 1 ? 1 : 0, {}, 1, plan_of<T>(c)};
+  if ((rc = edge_pairs(A.en.ue))) return rc;
   MarchArgs<T> ma{};
   ma.g = g;
   ma.P = P;
@@ -2324,7 +2389,7 @@ int stage_args(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v,
   ma.S = (c->nx + kStripOut - 1) / kStripOut;
   ma.shared = c->shared;
   ma.gsym = c->gsym;
-  ma.sink = edge_buf<T>(c, 3);
+  ma.sink = edge_buf<T>(c, kEdgeSink);
   A.ma = ma;
   A.ma.R = c->rows_a;
   A.ma.nbands = (c->nx + A.ma.R - 1) / A.ma.R;
@@ -2344,16 +2409,23 @@ int stage_args(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v,
   return SW_OK;
 }
 
-// One RK stage from (h, hp, u, v, up, vp) into (h_out, u_out, v_out), halos
-// filled.  Leaves the averaged edge velocities of this stage in evel.
+// One RK stage (stage index st of an rk-stage substep) from (h, hp, u, v, up,
+// vp) into (h_out, u_out, v_out), halos filled.  The stage's averaged edge
+// values come from the previous stage's last launch, or from a launch of
+// their own when `own_edge` (the first stage of a run).  The last launch
+// also forms the next stage's edge values from this stage's outputs.
 template <typename T>
 int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, const T* up,
-              const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last) {
+              const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last, int st,
+              int rk, bool own_edge) {
   StageArgs<T> A;
-  int rc = stage_args<T>(c, h, hp, u, v, up, vp, h_out, u_out, v_out, dt, beta, last, A);
+  int rc = stage_args<T>(c, h, hp, u, v, up, vp, h_out, u_out, v_out, dt, beta, last, st, rk, A);
   if (rc) return rc;
-  if ((rc = launch(c, K_EDGE, k_edge<T>, dim3(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))), A.e))) return rc;
+  const int ne = static_cast<int>(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx)));
+  if (own_edge && (rc = launch(c, K_EDGE, k_edge<T>, dim3(ne), A.e))) return rc;
   const dim3 ga((A.ma.nwaves + 3) / 4), gb((A.mb.nwaves + 3) / 4), gh(blocks_for(c->plan_n));
+  const int nh = static_cast<int>(blocks_for(c->plan_n));
+  auto tail = [&]() { return launch(c, K_HALO_UV, k_halo_uv_edge<T>, dim3(nh + ne), A.huv, A.en, nh); };
   if (c->rows_f > 0) {
     // fused height + deep-cell velocity, new-height halo, frame velocity
     const bool sf = A.mf.hp == A.mf.h && A.mf.up == A.mf.u && A.mf.vp == A.mf.v;
@@ -2369,7 +2441,7 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
     rc = c->gsym ? launch(c, K_FRAME, k_frame<T, true>, gr, A.mf, A.hh, nft)
                  : launch(c, K_FRAME, k_frame<T, false>, gr, A.mf, A.hh, nft);
     if (rc) return rc;
-    return launch(c, K_HALO_UV, k_halo_vector<T>, gh, A.huv);
+    return tail();
   }
   const bool sa = A.ma.hp == A.ma.h, sb = A.mb.up == A.mb.u && A.mb.vp == A.mb.v;
   auto la = [&](auto k) { return launch(c, K_MARCH_A, k, ga, A.ma); };
@@ -2385,16 +2457,16 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
   else
     rc = sb ? lb(k_march_b<T, false, true>) : lb(k_march_b<T, false, false>);
   if (rc) return rc;
-  return launch(c, K_HALO_UV, k_halo_vector<T>, gh, A.huv);
+  return tail();
 }
 
 template <typename T>
-int run_diag(const sw_ctx* c, const T* h, const T* u, const T* v, void* const out[13]) {
+int run_diag(const sw_ctx* c, const T* h, const T* u, const T* v, void* const out[13], int st) {
   DiagArgs<T> d{};
   d.g = geo_of(c);
   d.P = planes_of<T>(c);
   d.h = h; d.u = u; d.v = v;
-  d.evel = edge_buf<T>(c, 0);
+  d.evel = edge_buf<T>(c, 2 * st);  // the averaged edge velocity of that stage
   T** dst[10] = {&d.cx, &d.cy, &d.uct, &d.vct, &d.ucx, &d.ucy, &d.vcx, &d.vcy, &d.advx, &d.advy};
   for (int k = 0; k < 10; ++k) *dst[k] = static_cast<T*>(out[3 + k]);
   int rc = launch(c, K_DIAG, k_diag1<T>, dim3(blocks_for(c->cells)), d);
@@ -2427,18 +2499,20 @@ int check_disjoint(const sw_ctx* c, const void* const in[6], void* const out[13]
 // the last stage afterwards (the diagnostics are computed from them).
 template <typename T>
 int run_substep(const sw_ctx* c, int rk, double dtb, const T* h, const T* hp, const T* u,
-                const T* v, const T* up, const T* vp, T* const dst[3]) {
+                const T* v, const T* up, const T* vp, T* const dst[3], bool first = false) {
   T* s1[3] = {scratch_field<T>(c, 0), scratch_field<T>(c, 1), scratch_field<T>(c, 2)};
   T* s2[3] = {scratch_field<T>(c, 3), scratch_field<T>(c, 4), scratch_field<T>(c, 5)};
   int rc;
   if (rk == 2) {
-    if ((rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 2.0, 0.0, 0))) return rc;
-    return run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, dst[0], dst[1], dst[2], dtb, 0.0, 1);
+    if ((rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 2.0, 0.0, 0, 0, 2, first)))
+      return rc;
+    return run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, dst[0], dst[1], dst[2], dtb, 0.0, 1, 1, 2, false);
   }
-  if ((rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 3.0, 1.0, 0))) return rc;
-  if ((rc = run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, s2[0], s2[1], s2[2], dtb / 2.0, 1.0, 0)))
+  if ((rc = run_stage<T>(c, h, hp, u, v, up, vp, s1[0], s1[1], s1[2], dtb / 3.0, 1.0, 0, 0, 3, first)))
     return rc;
-  return run_stage<T>(c, s2[0], hp, s2[1], s2[2], up, vp, dst[0], dst[1], dst[2], dtb, 1.0, 1);
+  if ((rc = run_stage<T>(c, s1[0], hp, s1[1], s1[2], up, vp, s2[0], s2[1], s2[2], dtb / 2.0, 1.0, 0, 1, 3, false)))
+    return rc;
+  return run_stage<T>(c, s2[0], hp, s2[1], s2[2], up, vp, dst[0], dst[1], dst[2], dtb, 1.0, 1, 2, 3, false);
 }
 
 constexpr int kGraphPairs = 16;  // substep pairs recorded in one graph
@@ -2526,7 +2600,7 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   if (inplace && !((flags & SW_GRAPH) && nbb >= 3 && !c->prof)) {
     // outputs may alias inputs: the inputs are read by the first substep only,
     // so later substeps may write the outputs (the last one does)
-    if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, X))) return rc;
+    if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, X, true))) return rc;
     T** cur = X;
     for (int k = 1; k < nbb; ++k) {
       T** nxt = (k == nbb - 1) ? fin : (cur == X) ? Y : X;
@@ -2537,7 +2611,7 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
       for (int k = 0; k < 3; ++k)
         SW_HIP(hipMemcpyAsync(fin[k], cur[k], bytes, hipMemcpyDeviceToDevice, c->stream));
   } else if ((flags & SW_GRAPH) && nbb >= 3 && !c->prof) {
-    if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, X))) return rc;
+    if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, X, true))) return rc;
     const int pairs = (nbb - 1) / 2;
     int done = 0;
     while (done < pairs) {
@@ -2557,13 +2631,13 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   } else {
     for (int k = 0; k < nbb; ++k) {
       T** dst = ((nbb - 1 - k) % 2 == 0) ? fin : X;
-      if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, dst))) return rc;
+      if ((rc = run_substep<T>(c, rk, dtb, h, hp, u, v, up, vp, dst, k == 0))) return rc;
       h = hp = dst[0];
       u = up = dst[1];
       v = vp = dst[2];
     }
   }
-  if (!(flags & SW_SKIP_DIAG)) return run_diag<T>(c, dh, du, dv, out);
+  if (!(flags & SW_SKIP_DIAG)) return run_diag<T>(c, dh, du, dv, out, rk - 1);
   return SW_OK;
 }
 
@@ -2581,9 +2655,9 @@ template <typename T>
 int alloc_workspace(sw_ctx* c) {
   const size_t n = c->cells;
   SW_HIP(hipMalloc(&c->scratch, 12 * n * sizeof(T)));
-  SW_HIP(hipMalloc(&c->edges, 4 * edge_stride(c) * sizeof(T)));
+  SW_HIP(hipMalloc(&c->edges, kEdgeBufs * edge_stride(c) * sizeof(T)));
   SW_HIP(hipMemsetAsync(c->scratch, 0, 12 * n * sizeof(T), c->stream));
-  SW_HIP(hipMemsetAsync(c->edges, 0, 4 * edge_stride(c) * sizeof(T), c->stream));
+  SW_HIP(hipMemsetAsync(c->edges, 0, kEdgeBufs * edge_stride(c) * sizeof(T), c->stream));
   return SW_OK;
 }
 
@@ -2813,16 +2887,16 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
       auto at = [&](int st) -> StageArgs<T>& { return row[(static_cast<size_t>(st) * 4 + e) * K + q]; };
       if (rk == 2) {
         rc = stage_args<T>(c, src[0], src[1], src[2], src[3], src[4], src[5], S1[0], S1[1], S1[2],
-                           dtb / 2.0, 0.0, 0, at(0));
+                           dtb / 2.0, 0.0, 0, 0, 2, at(0));
         if (!rc) rc = stage_args<T>(c, S1[0], src[1], S1[1], S1[2], src[4], src[5], dst[0], dst[1],
-                                    dst[2], dtb, 0.0, 1, at(1));
+                                    dst[2], dtb, 0.0, 1, 1, 2, at(1));
       } else {
         rc = stage_args<T>(c, src[0], src[1], src[2], src[3], src[4], src[5], S1[0], S1[1], S1[2],
-                           dtb / 3.0, 1.0, 0, at(0));
+                           dtb / 3.0, 1.0, 0, 0, 3, at(0));
         if (!rc) rc = stage_args<T>(c, S1[0], src[1], S1[1], S1[2], src[4], src[5], S2[0], S2[1],
-                                    S2[2], dtb / 2.0, 1.0, 0, at(1));
+                                    S2[2], dtb / 2.0, 1.0, 0, 1, 3, at(1));
         if (!rc) rc = stage_args<T>(c, S2[0], src[1], S2[1], S2[2], src[4], src[5], dst[0], dst[1],
-                                    dst[2], dtb, 1.0, 1, at(2));
+                                    dst[2], dtb, 1.0, 1, 2, 3, at(2));
       }
       if (rc) return rc;
     }
@@ -2830,7 +2904,9 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     const int blocks[BK_CLASSES] = {static_cast<int>(blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))),
                                     (A.ma.nwaves + 3) / 4, static_cast<int>(blocks_for(c->plan_n)),
                                     (A.mb.nwaves + 3) / 4, (A.mf.nwaves + 3) / 4,
-                                    6 * frame_tiles(c->nx) + static_cast<int>(blocks_for(c->plan_n))};
+                                    6 * frame_tiles(c->nx) + static_cast<int>(blocks_for(c->plan_n)),
+                                    static_cast<int>(blocks_for(c->plan_n) +
+                                                     blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx)))};
     for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
   }
   // byte layout of the device copy
@@ -2900,7 +2976,10 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
         l.first_sub = (k == 0);
         l.parity = k & 1;
       }
-      hipLaunchKernelGGL(k_edge_batch<T>, dim3(first[BK_EDGE][active]), dim3(256), 0, s, L[BK_EDGE]);
+      // edge values: the first stage's own pass; later stages' come from the
+      // previous stage's last launch
+      if (k == 0 && st == 0)
+        hipLaunchKernelGGL(k_edge_batch<T>, dim3(first[BK_EDGE][active]), dim3(256), 0, s, L[BK_EDGE]);
       // stage 1 of a substep reads its *p fields at the addresses of h, u, v
       // for every member, except on the first substep unless every member's
       // input aliases them
@@ -2921,7 +3000,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
           hipLaunchKernelGGL((k_frame_batch<T, true>), gr, dim3(256), 0, s, L[BK_FRAME]);
         else
           hipLaunchKernelGGL((k_frame_batch<T, false>), gr, dim3(256), 0, s, L[BK_FRAME]);
-        hipLaunchKernelGGL(k_halo_vector_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, s, L[BK_HALO]);
+        hipLaunchKernelGGL(k_halo_uv_edge_batch<T>, dim3(first[BK_HUVE][active]), dim3(256), 0, s, L[BK_HUVE]);
         SW_HIP(hipGetLastError());
         continue;
       }
@@ -2942,7 +3021,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
         hipLaunchKernelGGL((k_march_b_batch<T, false, true>), gb, dim3(256), 0, s, L[BK_MARCH_B]);
       else
         hipLaunchKernelGGL((k_march_b_batch<T, false, false>), gb, dim3(256), 0, s, L[BK_MARCH_B]);
-      hipLaunchKernelGGL(k_halo_vector_batch<T>, dim3(first[BK_HALO][active]), dim3(256), 0, s, L[BK_HALO]);
+      hipLaunchKernelGGL(k_halo_uv_edge_batch<T>, dim3(first[BK_HUVE][active]), dim3(256), 0, s, L[BK_HUVE]);
       SW_HIP(hipGetLastError());
     }
     return SW_OK;
@@ -3005,7 +3084,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
       hipStream_t keep = c->stream;
       c->stream = stream;
       const int rc = run_diag<T>(c, scratch_field<T>(c, ls), scratch_field<T>(c, ls + 1),
-                                 scratch_field<T>(c, ls + 2), m.out);
+                                 scratch_field<T>(c, ls + 2), m.out, rk - 1);
       c->stream = keep;
       if (rc) return rc;
     }
@@ -3114,15 +3193,17 @@ int sw_stage(sw_ctx* ctx, const void* const in[6], void* const out[13], double d
   if (ctx->dtype == SW_F64) {
     auto p = [&](int k) { return static_cast<const double*>(in[k]); };
     rc = run_stage<double>(ctx, p(0), p(1), p(2), p(3), p(4), p(5), static_cast<double*>(out[0]),
-                           static_cast<double*>(out[1]), static_cast<double*>(out[2]), dt, beta, is_last ? 1 : 0);
+                           static_cast<double*>(out[1]), static_cast<double*>(out[2]), dt, beta, is_last ? 1 : 0,
+                           0, 2, true);
     if (rc) return rc;
-    return run_diag<double>(ctx, p(0), p(2), p(3), out);
+    return run_diag<double>(ctx, p(0), p(2), p(3), out, 0);
   }
   auto p = [&](int k) { return static_cast<const float*>(in[k]); };
   rc = run_stage<float>(ctx, p(0), p(1), p(2), p(3), p(4), p(5), static_cast<float*>(out[0]),
-                        static_cast<float*>(out[1]), static_cast<float*>(out[2]), dt, beta, is_last ? 1 : 0);
+                        static_cast<float*>(out[1]), static_cast<float*>(out[2]), dt, beta, is_last ? 1 : 0,
+                        0, 2, true);
   if (rc) return rc;
-  return run_diag<float>(ctx, p(0), p(2), p(3), out);
+  return run_diag<float>(ctx, p(0), p(2), p(3), out, 0);
 }
 
 int sw_halo_scalar(sw_ctx* ctx, void* field, int ncomp) {
@@ -3146,12 +3227,12 @@ int sw_edge_avg(sw_ctx* ctx, void* u, void* v) {
   const size_t n = 24 * static_cast<size_t>(ctx->nx);
   if (ctx->dtype == SW_F64) {
     EdgeAvgArgs<double> a{geo_of(ctx), route(), static_cast<double*>(u), static_cast<double*>(v),
-                          edge_buf<double>(ctx, 2)};
+                          edge_buf<double>(ctx, kEdgeScratch)};
     hipLaunchKernelGGL(k_edge_avg_read<double>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_edge_avg_write<double>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
   } else {
     EdgeAvgArgs<float> a{geo_of(ctx), route(), static_cast<float*>(u), static_cast<float*>(v),
-                         edge_buf<float>(ctx, 2)};
+                         edge_buf<float>(ctx, kEdgeScratch)};
     hipLaunchKernelGGL(k_edge_avg_read<float>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_edge_avg_write<float>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, a);
   }
