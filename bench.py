@@ -17,7 +17,11 @@ The JSON line also carries
   roofline_path  the whole step against 296 B per FP64 cell-substep (148 B for
                  FP32) at 8 TB/s,
   cpu_baseline   the NumPy FP64 oracle on one host core over a bounded sample
-                 of the same workload.
+                 of the same workload,
+  serving        p50/p99 latency and throughput of served solves (a seeded
+                 Poisson trace of mixed resolutions, step counts and
+                 precisions under continuous batching) beside the plain FP64
+                 path serving the same trace one request at a time.
 Only one GPU is used: --gpus above 1 exits with a message.
 """
 
@@ -66,6 +70,8 @@ def parse():
     p.add_argument("--cpu-substeps", type=int, default=8,
                    help="substeps of the CPU oracle sample (0 disables)")
     p.add_argument("--cpu-nx", type=int, default=None, help="resolution of the CPU sample (default --nx)")
+    p.add_argument("--serve-requests", type=int, default=60,
+                   help="requests of the served trace (Poisson 200/s, C48-C768, 50-5000 steps; 0 disables)")
     return p.parse_args()
 
 
@@ -190,6 +196,23 @@ def main() -> int:
     if args.cpu_substeps > 0:
         cpu = cpu_baseline(args.cpu_nx or nx, args.cpu_substeps)
 
+    serving = None
+    if args.serve_requests > 0:
+        from licom4py_gpu_amd import serving as srv
+        rep = srv.serve_report(args.serve_requests, 200.0, [48, 96, 192, 384, 768], (50, 5000), 200.0, 100,
+                               7, dev, baselines=("fp64",))
+        base = rep["one_at_a_time_fp64"]
+        serving = {"workload": rep["workload"],
+                   "p50_s": round(rep["latency"]["p50_s"], 4), "p99_s": round(rep["latency"]["p99_s"], 4),
+                   "throughput_cell_substeps_per_s": rep["throughput_cell_substeps_per_s"],
+                   "mean_batch": round(rep["mean_batch"], 2), "peak_hbm_gb": round(rep["peak_hbm_gb"], 3),
+                   "plain_fp64_one_at_a_time": {"p50_s": round(base["p50_s"], 4), "p99_s": round(base["p99_s"], 4),
+                                                "throughput_cell_substeps_per_s":
+                                                    base["throughput_cell_substeps_per_s"]}}
+        if not rep["finite"]:
+            print("bench.py: non-finite served result", file=sys.stderr)
+            return 1
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -212,6 +235,7 @@ def main() -> int:
         "roofline_path": roofline_path,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -340,10 +340,21 @@ def main(argv=None) -> int:
     a = p.parse_args(argv)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    reqs = seeded_trace(a.requests, a.rate, a.res, (a.steps_min, a.steps_max), seed=a.seed, device=dev)
+    line = serve_report(a.requests, a.rate, a.res, (a.steps_min, a.steps_max), a.budget_gb, a.chunk,
+                        a.seed, dev, baselines=() if a.no_baseline else ("mixed", "fp64"))
+    print(json.dumps(line), flush=True)
+    return 0 if line["finite"] else 1
+
+
+def serve_report(requests: int, rate: float, res: Sequence[int], steps_range, budget_gb: float, chunk: int,
+                 seed: int, dev, baselines=("mixed", "fp64")) -> dict:
+    """Serve a seeded Poisson trace on a warm server and report latency and
+    throughput; `baselines` adds the plain path one request at a time with
+    each request's own precision ("mixed") and all in FP64 ("fp64")."""
+    reqs = seeded_trace(requests, rate, res, steps_range, seed=seed, device=dev)
     # make each resolution's consts resident and load the kernels before the
     # timed trace (a warm service), then reset the statistics
-    srv = Server(a.budget_gb * 1e9, a.chunk, dev)
+    srv = Server(budget_gb * 1e9, chunk, dev)
     warm = []
     for dt_ in (torch.float64, torch.float32):
         for nx in sorted(set(r.nx for r in reqs)):
@@ -356,23 +367,23 @@ def main(argv=None) -> int:
     makespan = time.perf_counter() - t0
     work = sum(6 * r.nx * r.nx * r.steps for r in reqs)
     line = {
-        "workload": f"{a.requests} requests, Poisson {a.rate}/s, C in {sorted(set(a.res))}, steps log-uniform "
-                    f"[{a.steps_min}, {a.steps_max}], FP32/FP64 half each, RK2",
+        "workload": f"{requests} requests, Poisson {rate}/s, C in {sorted(set(res))}, steps log-uniform "
+                    f"[{steps_range[0]}, {steps_range[1]}], FP32/FP64 half each, RK2",
         "offered_cell_substeps_per_s": work / (reqs[-1].arrival if reqs else 1.0),
         "latency": latency_stats(done),
         "makespan_s": makespan,
         "throughput_cell_substeps_per_s": work / makespan,
-        "ticks": srv.ticks, "chunk": a.chunk,
+        "ticks": srv.ticks, "chunk": chunk,
         "mean_batch": float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0,
         "max_batch": int(max(srv.batch_sizes)) if srv.batch_sizes else 0,
-        "peak_hbm_gb": srv.peak_used / 1e9, "budget_gb": a.budget_gb,
+        "peak_hbm_gb": srv.peak_used / 1e9, "budget_gb": budget_gb,
         "finite": bool(all(math.isfinite(r.checksum) for r in done)),
     }
-    if not a.no_baseline:
+    if "mixed" in baselines:
         line["one_at_a_time"] = one_at_a_time(reqs, srv.pool)
+    if "fp64" in baselines:
         line["one_at_a_time_fp64"] = one_at_a_time(reqs, srv.pool, force_fp64=True)
-    print(json.dumps(line), flush=True)
-    return 0 if line["finite"] else 1
+    return line
 
 
 if __name__ == "__main__":
