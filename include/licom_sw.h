@@ -134,11 +134,18 @@ int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nb
 int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, double out[2]);
 
 /* Rows per wavefront band of the stage kernels: rows_a / rows_b for the
- * separate height and velocity kernels, rows_f for the fused kernel, which
- * runs when rows_f > 0 (the default, 32); rows_f = 0 selects the separate
- * kernels.  Performance only: every setting in [4, 4096] gives bit-identical
- * results.  Batched runs (sw_batch_run) use the separate kernels. */
+ * separate height and velocity kernels (default 32), rows_f for the fused
+ * kernel, which runs when rows_f > 0; rows_f = 0 selects the separate
+ * kernels.  sw_create sets rows_f from the resolution and the device's CU
+ * count (the band height that keeps the fused waves within few rounds of
+ * resident waves).  Performance only: every setting in [4, 4096] gives
+ * bit-identical results.  A batch (sw_batch_run) uses the fused kernel when
+ * every member's rows_f > 0.  Build-defined, no counterpart in the
+ * reference. */
 int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b, int rows_f);
+
+/* The context's current band heights (sw_set_tuning).  Build-defined. */
+int sw_get_tuning(const sw_ctx* ctx, int* rows_a, int* rows_b, int* rows_f);
 
 /* Host-only: what sw_create detected in the consts.  Bit 0: dzph_x/dzph_y
  * uniform (read as scalars); bit 1: rdx, rdy, rda, c_dy, d_dx identical on all

@@ -24,7 +24,7 @@ SW_N_CONSTS = 17
 EXPORTED = (
     "sw_create", "sw_destroy", "sw_set_stream", "sw_rk2", "sw_rk3", "sw_stage",
     "sw_halo_scalar", "sw_halo_vector", "sw_edge_avg", "sw_sync",
-    "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_layout_flags", "sw_profile", "sw_integrals",
+    "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_get_tuning", "sw_layout_flags", "sw_profile", "sw_integrals",
     "sw_create_like", "sw_batch_create", "sw_batch_destroy", "sw_batch_run",
 )
 
@@ -82,6 +82,8 @@ def lib() -> ctypes.CDLL:
     L.sw_integrals.restype = ci
     L.sw_set_tuning.argtypes = [vp, ci, ci, ci]
     L.sw_set_tuning.restype = ci
+    L.sw_get_tuning.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]
+    L.sw_get_tuning.restype = ci
     L.sw_layout_flags.argtypes = [vp, ctypes.POINTER(ci)]
     L.sw_layout_flags.restype = ci
     L.sw_create_like.argtypes = [ctypes.POINTER(vp), vp]

@@ -22,7 +22,7 @@ from typing import Dict, Iterable, Optional, Sequence, Tuple
 
 CANDIDATES: Tuple[int, ...] = (8, 12, 16, 20, 24, 26, 28, 30, 32, 36, 40, 48)
 CACHE_ENV = "LICOM_SW_TUNE_CACHE"
-_KERNEL_SET = "fused1"  # cache entries of other kernel generations are ignored
+_KERNEL_SET = "fused2"  # cache entries of other kernel generations are ignored
 
 
 def _cache_path() -> str:

@@ -78,7 +78,7 @@ class SWContext:
     """Owns one `sw_ctx` (internal const planes, scratch) for a resolution and
     precision.  Launches go to the current torch stream of the call."""
 
-    tuning = (32, 32, 32)  # rows_a, rows_b, rows_f of the library's defaults
+    tuning = (32, 32, 32)  # rows_a, rows_b, rows_f (read from the library after sw_create)
 
     def __init__(self, consts: Sequence, dtype: torch.dtype = torch.float64):
         if consts is None:  # filled in by like()
@@ -100,6 +100,9 @@ class SWContext:
                                          _hip.SW_F64 if dtype == torch.float64 else _hip.SW_F32,
                                          arr, ctypes.c_void_p(stream)), "sw_create")
         self._h = handle
+        rows = [ctypes.c_int() for _ in range(3)]
+        _hip.check(_hip.lib().sw_get_tuning(handle, *(ctypes.byref(r) for r in rows)), "sw_get_tuning")
+        self.tuning = tuple(r.value for r in rows)
 
     def like(self) -> "SWContext":
         """Another context of this resolution and precision with its own

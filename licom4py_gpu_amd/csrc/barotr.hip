@@ -511,10 +511,15 @@ __device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRa
 // Inputs of one march step: the column point of row crow, the i-face of row
 // i, the cell of row ii and the halo column of row hrow.  SAMEP: hp is h
 // (stage 1 of a substep that starts from its own output), so hp at row ii is
-// already in the lane's h window and is not loaded.
+// already in the lane's h window and is not loaded.  Loads whose values no
+// lane of the wave uses are skipped behind wave-uniform tests: the cube-edge
+// velocity and flux of the i-face off the edge rows and of the j-face in
+// strips without an edge column, the depth where it is uniform (the scalar
+// argument holds it).  hl: this lane loads the halo column (false: p.hr and
+// p.hh are left unset and the caller does not use them).
 template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w, int i, int crow, int ii,
-                                        int hrow, int hcol, APre<T>& p) {
+                                        int hrow, int hcol, APre<T>& p, bool hl = true) {
   const int N = a.g.N, nx = a.g.nx;
   const Planes<T>& P = a.P;
   const uint32_t NS = static_cast<uint32_t>(N);
@@ -524,23 +529,36 @@ __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w
   p.h = ldg(a.h, oc);
   const uint32_t oi = static_cast<uint32_t>(min(i, N - 1)) * NS + jl;
   p.cdy = ldg(P.cdy, w.mo + oi);
-  p.dzx = ldg(P.dzx, P.dz_uniform ? 0 : w.fo + oi);
-  const bool iedge = (i == HALO || i == N - HALO) && j >= HALO && j <= N - 4;
-  const int ie = iedge ? (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO) : 0;
-  p.evi = ldg(a.evel, ie);
-  p.efi = ldg(a.eflux, ie);
   const uint32_t om = static_cast<uint32_t>(ii) * NS + jl;
+  if (P.dz_uniform) {
+    p.dzx = P.dzx_s;
+    p.dzy = P.dzy_s;
+  } else {
+    p.dzx = ldg(P.dzx, w.fo + oi);
+    p.dzy = ldg(P.dzy, w.fo + om);
+  }
+  p.evi = p.efi = T(0);
+  if (i == HALO || i == N - HALO) {
+    const bool iedge = j >= HALO && j <= N - 4;
+    const int ie = iedge ? (t * 4 + (i == HALO ? DS : DN)) * nx + (j - HALO) : 0;
+    p.evi = ldg(a.evel, ie);
+    p.efi = ldg(a.eflux, ie);
+  }
   p.rda = ldg(P.rda, w.mo + om);
   p.ddx = ldg(P.ddx, w.mo + om);
   if constexpr (!SAMEP) p.hp = ldg(a.hp, w.fo + om);
-  p.dzy = ldg(P.dzy, P.dz_uniform ? 0 : w.fo + om);
-  const bool jedge = (j == HALO || j == N - HALO) && ii >= HALO && ii < N - HALO;
-  const int je = jedge ? (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
-  p.evj = ldg(a.evel, je);
-  p.efj = ldg(a.eflux, je);
-  const uint32_t oh = w.fo + static_cast<uint32_t>(hrow) * NS + hcol;
-  load_ct2<T, GSYM>(a, oh, p.hr);
-  p.hh = ldg(a.h, oh);
+  p.evj = p.efj = T(0);
+  if ((w.c0 <= HALO && HALO <= w.c0 + 63) || (w.c0 <= N - HALO && N - HALO <= w.c0 + 63)) {
+    const bool jedge = (j == HALO || j == N - HALO) && ii >= HALO && ii < N - HALO;
+    const int je = jedge ? (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
+    p.evj = ldg(a.evel, je);
+    p.efj = ldg(a.eflux, je);
+  }
+  if (hl) {
+    const uint32_t oh = w.fo + static_cast<uint32_t>(hrow) * NS + hcol;
+    load_ct2<T, GSYM>(a, oh, p.hr);
+    p.hh = ldg(a.h, oh);
+  }
 }
 
 // Kernel A's step at face i (marching down): column point row i+2, cell row i-1.
@@ -967,14 +985,18 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
 // Fused stage kernel: kernel A's height march and kernel B's velocity march in
 // one pass over the column loads.
 //
-// A wave covers 64 columns c0 = 57 s .. c0 + 63 of a band and computes the new
-// height on lanes 0..62 (kernel A's expressions).  Kernel B's pressure
-// gradient needs the new height at +-3 rows and columns, so the velocity is
-// written on lanes 3..59 at cells whose 7 x 7 new-height stencil lies inside
-// the face ("deep" cells: rows and columns 6 .. N-7); the 3-cell frame along
-// the face edges needs the new height's halo from the neighbouring faces and
-// is done by k_frame after the halo fill.  Rows of the band: the march covers
-// r0-3 .. r1+2 (clipped to the interior) and writes the height on r0 .. r1-1.
+// A wave of strip s loads the 64 columns c0 = 52 s - 3 .. c0 + 63 of a band
+// (clamped to the face) and nothing outside them: a j-face reads the six
+// columns around it, so the new height (kernel A's expressions) is valid on
+// lanes 3..60, and kernel B's pressure gradient needs the new height at +-3
+// rows and columns, so the velocity is valid on lanes 6..57.  The wave writes
+// both on lanes 6..57 (columns 52 s + 3 .. 52 s + 54); the neighbouring strips
+// recompute the three columns on either side.  The velocity is written at
+// cells whose 7 x 7 new-height stencil lies inside the face ("deep" cells:
+// rows and columns 6 .. N-7); the 3-cell frame along the face edges needs the
+// new height's halo from the neighbouring faces and is done by k_frame after
+// the halo fill.  Rows of the band: the march covers r0-3 .. r1+2 (clipped to
+// the interior) and writes the height on r0 .. r1-1.
 //
 // Kernel B evaluates, on each i-face and j-face, the same upwind velocity,
 // celerity and face values of h and of the normal velocity as kernel A's mass
@@ -991,11 +1013,10 @@ __global__ __launch_bounds__(256) void k_march_b(MarchArgs<T> a) {
 // contravariant u, h and covariant v; and the blended new height of the
 // lane's column over the seven rows the pressure gradient of row ib reads.
 // LDS per wave: row buffers of contravariant v, covariant u and h (4 rows,
-// written with their halo columns when the row is loaded, three rows ahead
-// of its cell row, as the neighbouring strip loads the same lines), the
+// written when the row is loaded, three rows ahead of its cell row), the
 // blended new height of row ia and the FIFO (3 rows).
 // ---------------------------------------------------------------------------
-constexpr int kFStride = 57;  // deep-cell columns per wave and row
+constexpr int kFStride = 52;  // columns a wave writes per row (lanes 6..57)
 constexpr int kRing = 4;
 constexpr int kFifo = 3;
 constexpr int kFTerms = 5;    // advx cor_u visx u rdx of the u update
@@ -1050,9 +1071,9 @@ __device__ __forceinline__ bool decode_fwave(const MarchArgs<T>& a, int L, WaveT
   const int rest = wid / 6;
   const int s = rest % a.S;
   const int band = rest / a.S;
-  w.c0 = kFStride * s;
+  w.c0 = kFStride * s - HALO;
   w.j = w.c0 + w.lane;
-  w.jl = min(w.j, N - 1);
+  w.jl = min(max(w.j, 0), N - 1);
   w.r0 = HALO + band * a.R;
   w.r1 = min(w.r0 + a.R, N - HALO);
   const uint32_t NN = static_cast<uint32_t>(N) * N;
@@ -1081,14 +1102,10 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   constexpr int SA = UP ? 3 : 2;
   constexpr int STEP = UP ? -1 : 1;
 
-  // halo columns of the row buffers: c0-3 .. c0-1 (lanes 0..2), c0+64, c0+65
-  // (lanes 3, 4); a clamped column rewrites the slot of its own column with
-  // the same value
-  int hcol = lane < 3 ? c0 - 3 + lane : c0 + 61 + lane;
-  const bool halo_lane = lane < 5;
-  hcol = halo_lane ? min(max(hcol, 0), N - 1) : w.jl;
-  const int hslot = halo_lane ? hcol - c0 + 3 : lane + 3;
-  const bool own = lane >= 3 && lane < 3 + kFStride && j <= N - 1 - HALO;
+  // the lanes load columns c0 .. c0+63 (clamped to the face); the new height
+  // is valid on lanes 3..60 and the velocity on lanes 6..57, the columns the
+  // wave writes
+  const bool own = lane >= 2 * HALO && lane < 2 * HALO + kFStride && j >= HALO && j <= N - 1 - HALO;
   const bool deep = own && j >= 2 * HALO && j <= N - 1 - 2 * HALO;
   const int a0 = max(w.r0 - 3, HALO), a1 = min(w.r1 + 3, N - HALO);
   const int b0 = max(w.r0, 2 * HALO), b1 = min(w.r1, N - 2 * HALO);
@@ -1098,7 +1115,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   // column, i-face i, cell row ia
   auto fetch = [&](int i, APre<T>& p) {
     const int rn = UP ? max(i - 3, 0) : min(i + 2, N - 1);
-    a_fetch<T, GSYM, SAMEP>(a, w, i, rn, UP ? min(i, N - 1) : max(i - 1, 0), rn, hcol, p);
+    a_fetch<T, GSYM, SAMEP>(a, w, i, rn, UP ? min(i, N - 1) : max(i - 1, 0), rn, w.jl, p, false);
   };
   auto shift = [](T* x) {
     if (UP) {
@@ -1109,16 +1126,12 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       for (int m = 0; m < 5; ++m) x[m] = x[m + 1];
     }
   };
-  // row buffers of the newest row: contravariant v, covariant u and h with
-  // the halo columns
-  auto put_row = [&](int row, const CtRaw<T>& c, T hc, const CtRaw<T>& hr, T hh_) {
+  // row buffers of the newest row: contravariant v, covariant u and h
+  auto put_row = [&](int row, const CtRaw<T>& c, T hc) {
     const int s = row & (kRing - 1);
     rg.v[s][lane + 3] = ct_v(c);
     rg.u[s][lane + 3] = c.u;
     rg.h[s][lane + 3] = hc;
-    rg.v[s][hslot] = ct_v(hr);
-    rg.u[s][hslot] = hr.u;
-    rg.h[s][hslot] = hh_;
   };
 
   T cu[6], hh[6], cv[6], wn7[7];
@@ -1130,15 +1143,12 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     const int slot = UP ? m + 1 : m;
     const int row = f0 - 3 + slot;
     const uint32_t o = w.fo + static_cast<uint32_t>(row) * NS;
-    CtRaw<T> r, hr;
+    CtRaw<T> r;
     load_ct2<T, GSYM>(a, o + w.jl, r);
     cu[slot] = ct_u(r);
     cv[slot] = r.v;
     hh[slot] = ldg(a.h, o + w.jl);
-    if (UP ? m < 2 : m >= 2) {
-      load_ct2<T, GSYM>(a, o + hcol, hr);
-      put_row(row, r, hh[slot], hr, ldg(a.h, o + hcol));
-    }
+    if (UP ? m < 2 : m >= 2) put_row(row, r, hh[slot]);
   }
 #pragma unroll
   for (int m = 0; m < 7; ++m) wn7[m] = T(0);
@@ -1153,7 +1163,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     cu[NEW] = ct_u(cur.cr);
     cv[NEW] = cur.cr.v;
     hh[NEW] = cur.h;
-    put_row(UP ? f0 - 3 : f0 + 2, cur.cr, cur.h, cur.hr, cur.hh);
+    put_row(UP ? f0 - 3 : f0 + 2, cur.cr, cur.h);
     const bool edge = (f0 == HALO) || (f0 == N - HALO);
     const FaceQ<T> q = face_q(hh, cu, cv, edge, cur.evi, cur.dzx);
     Fprev = (a.last && edge) ? cur.efi : q.F;
@@ -1197,7 +1207,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     cu[NEW] = ct_u(cur.cr);
     cv[NEW] = cur.cr.v;
     hh[NEW] = cur.h;
-    put_row(UP ? i - 3 : i + 2, cur.cr, cur.h, cur.hr, cur.hh);
+    put_row(UP ? i - 3 : i + 2, cur.cr, cur.h);
 
     // i-face at row i: mass flux and velocity terms
     const bool iedge = (i == HALO) || (i == N - HALO);
@@ -2188,7 +2198,7 @@ struct sw_ctx {
   int plan_n = 0;
   int rows_a = 32;         // rows per wave of k_march_a
   int rows_b = 32;         // rows per wave of k_march_b
-  int rows_f = 32;         // rows per wave of k_fused; 0 selects k_march_a + k_march_b
+  int rows_f = 32;         // rows per wave of k_fused (sw_create sets default_rows_f); 0 selects k_march_a + k_march_b
   hipStream_t capture = nullptr;  // private stream used only to record graphs
   struct GraphEntry {
     int rk, pairs;
@@ -3096,6 +3106,25 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
 
 extern "C" {
 
+// Default band height of k_fused: the height that minimises (rounds of
+// resident waves) x (rows one wave marches, its band plus the 6 rows of
+// overlap), with two 4-wave blocks resident per CU.
+int default_rows_f(int nx, int cus) {
+  const long S = (nx + kFStride - 1) / kFStride;
+  const long slots = 8L * (cus > 0 ? cus : 256);
+  int best = 32;
+  long best_cost = -1;
+  for (int R : {8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64}) {
+    const long waves = 6 * S * ((nx + R - 1) / R);
+    const long cost = ((waves + slots - 1) / slots) * (std::min(R, nx) + 6);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = R;
+    }
+  }
+  return best;
+}
+
 int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CONSTS], void* stream) {
   if (!out) return fail(SW_E_ARG, "out is null");
   *out = nullptr;
@@ -3109,6 +3138,13 @@ int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CON
   c->dtype = dtype;
   c->stream = static_cast<hipStream_t>(stream);
   c->cells = 6 * static_cast<size_t>(c->N) * c->N;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 0;
+    c->rows_f = default_rows_f(nx, cus);
+  }
   rc = validate_consts(c, consts);
   if (!rc) rc = (dtype == SW_F64) ? do_create_planes<double>(c, consts) : do_create_planes<float>(c, consts);
   if (!rc) {
@@ -3288,6 +3324,14 @@ int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, doubl
   if (!ctx || !h || !u || !v || !out) return fail(SW_E_ARG, "null argument");
   return ctx->dtype == SW_F64 ? run_integrals<double>(ctx, h, u, v, out)
                               : run_integrals<float>(ctx, h, u, v, out);
+}
+
+int sw_get_tuning(const sw_ctx* ctx, int* rows_a, int* rows_b, int* rows_f) {
+  if (!ctx || !rows_a || !rows_b || !rows_f) return fail(SW_E_ARG, "null argument");
+  *rows_a = ctx->rows_a;
+  *rows_b = ctx->rows_b;
+  *rows_f = ctx->rows_f;
+  return SW_OK;
 }
 
 int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b, int rows_f) {

@@ -107,7 +107,7 @@ def test_stage_matches_golden_fixture():
         _assert_exact(f"golden {name}", _host(gv), want[name])
 
 
-@pytest.mark.parametrize("nx", [7, 8, 13, 48, 57, 58, 70, 140])
+@pytest.mark.parametrize("nx", [7, 8, 13, 48, 52, 53, 57, 58, 70, 104, 105, 140])
 @pytest.mark.parametrize("last", [False, True])
 @pytest.mark.parametrize("beta", [0.0, 1.0])
 def test_stage_matches_oracle(nx, last, beta):
@@ -242,6 +242,21 @@ def test_band_height_does_not_change_results(rows):
     got = ctx.stage(*[_dev(x, torch) for x in core], 20.0, 0.0, True)
     for name, gv, wv in zip(NAMES13, got, want):
         _assert_exact(f"rows={rows} {name}", _host(gv), wv)
+
+
+def test_default_tuning_read_back():
+    """sw_create picks the fused kernel's band height from the resolution and
+    the device; the context reports what the library holds."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    for nx in (48, 140):
+        _, consts = grid_consts(nx)
+        ctx = barotr.SWContext(consts, torch.float64)
+        ra, rb, rf = ctx.tuning
+        assert (ra, rb) == (32, 32) and rf in (8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64)
+        ctx.set_tuning(16, 20, 0)
+        assert ctx.tuning == (16, 20, 0)
+        assert ctx.like().tuning == (16, 20, 0)
 
 
 @pytest.mark.parametrize("graph,rows_f", [(True, 32), (False, 32), (True, 0), (False, 0)])
