@@ -94,14 +94,24 @@ class GridPool:
     """Per (resolution, dtype): the grid, its device consts and a base context
     whose const planes later members share.  With ``tune`` the base context's
     band heights are picked by the autotuner (on the seeded field of that
-    resolution, cached per device) and inherited by every member context."""
+    resolution, cached per device) and inherited by every member context;
+    else ``rows`` = (rows_a, rows_b, rows_f), if given, replaces the library's
+    per-resolution defaults (which suit one context alone on the device)."""
 
-    def __init__(self, device=None, tune: bool = False):
+    def __init__(self, device=None, tune: bool = False, rows=None):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.tune = tune
+        self.rows = tuple(rows) if rows is not None else None
         self.tuning: Dict[tuple, dict] = {}
         self._consts: Dict[int, tuple] = {}
         self._base: Dict[tuple, "barotr.SWContext"] = {}
+
+    def sibling(self, rows=None) -> "GridPool":
+        """A pool on the same device sharing this pool's grids and device
+        consts, with its own contexts and band heights."""
+        other = GridPool(self.device, rows=rows)
+        other._consts = self._consts
+        return other
 
     def consts(self, nx: int):
         if nx not in self._consts:
@@ -121,6 +131,8 @@ class GridPool:
                         for x in initial.state_tuple(h, u, v)[:6]]
                 r = autotune.tune(ctx, core, initial.courant_dt(g, h, sp))
                 self.tuning[key] = {"rows_a": r["rows_a"], "rows_b": r["rows_b"], "rows_f": r["rows_f"]}
+            elif self.rows is not None:
+                ctx.set_tuning(*self.rows)
             self._base[key] = ctx
             return ctx
         return self._base[key].like()

@@ -92,18 +92,24 @@ def counter_stats(db_path: str):
     return {k: (len(v), sum(v.values()) / len(v)) for k, v in acc.items()}
 
 
-def dispatch_phases(db_path: str, kernel: str, tail: int):
+def dispatch_phases(db_path: str, kernel: str, tail: int, prev: int = 0):
     """Average duration (us) of the kernel's dispatches: all of them, the last
-    `tail` (bench.py's event-timed eager pass runs last), and the others."""
+    `tail` (bench.py's event-timed eager pass runs last), the `prev` before
+    those (bench.py's timed steps) and the others."""
     db = sqlite3.connect(db_path)
     rows = db.execute("select duration from kernels where name like ? order by start",
                       (f"%{kernel}%",)).fetchall()
     d = [r[0] / 1e3 for r in rows]
     if not d:
         return None
-    head, last = d[:-tail] if tail < len(d) else [], d[-tail:]
+    tail = min(tail, len(d))
+    prev = min(prev, len(d) - tail)
+    last = d[len(d) - tail:]
+    before = d[len(d) - tail - prev:len(d) - tail]
+    head = d[:len(d) - tail - prev]
     mean = lambda x: sum(x) / len(x) if x else float("nan")  # noqa: E731
-    return {"all": (len(d), mean(d)), "last": (len(last), mean(last)), "others": (len(head), mean(head))}
+    return {"all": (len(d), mean(d)), "last": (len(last), mean(last)),
+            "before": (len(before), mean(before)), "others": (len(head), mean(head))}
 
 
 def _summarize(args) -> int:
@@ -129,13 +135,18 @@ def _summarize(args) -> int:
             for (k, c), (n, m) in sorted(cs.items()):
                 lines.append(f"| {k} | {c} | {n} | {m:.1f} |")
         for spec in args.phases or []:
-            kname, tail = spec.rsplit(":", 1)
-            ph = dispatch_phases(path, kname, int(tail))
+            parts = spec.split(":")
+            kname, tail = parts[0], int(parts[1])
+            prev = int(parts[2]) if len(parts) > 2 else 0
+            ph = dispatch_phases(path, kname, tail, prev)
             if ph:
-                lines.append(f"Dispatches of {kname}: all {ph['all'][0]} avg {ph['all'][1]:.3f} us; "
-                             f"last {ph['last'][0]} (the event-timed eager pass) avg {ph['last'][1]:.3f} us; "
-                             f"the others (graph replays of the timed steps, tuner passes) "
-                             f"{ph['others'][0]} avg {ph['others'][1]:.3f} us")
+                text = (f"Dispatches of {kname}: all {ph['all'][0]} avg {ph['all'][1]:.3f} us; "
+                        f"last {ph['last'][0]} (the event-timed eager pass) avg {ph['last'][1]:.3f} us; ")
+                if prev:
+                    text += (f"the {ph['before'][0]} before them (graph replays of the timed steps) "
+                             f"avg {ph['before'][1]:.3f} us; ")
+                text += f"the others (tuner passes, warm-up, served solves) {ph['others'][0]} avg {ph['others'][1]:.3f} us"
+                lines.append(text)
         lines.append("")
     text = "\n".join(lines)
     if args.out:
@@ -194,7 +205,8 @@ def main(argv=None) -> int:
     s.add_argument("db", nargs="+")
     s.add_argument("--out", default=None)
     s.add_argument("--command-line", default="")
-    s.add_argument("--phases", nargs="*", help="KERNEL:N - split the KERNEL dispatches into the last N and the rest")
+    s.add_argument("--phases", nargs="*",
+                   help="KERNEL:N[:M] - split the KERNEL dispatches into the last N, the M before them and the rest")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
