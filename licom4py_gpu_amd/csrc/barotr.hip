@@ -1177,7 +1177,10 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   }
 
   int sq = 0;  // FIFO slot of rows ia and ib (read before written)
-  for (int n = 1; n <= a1 - a0; ++n) {
+  // step n reads the inputs fetched by the step before (cur) and fetches the
+  // next step's (nxt); the loop alternates two buffers, so no copy between
+  // them is needed
+  auto step = [&](int n, const APre<T>& cur, APre<T>& nxt) __attribute__((always_inline)) {
     const int i = f0 + STEP * n;
     const int ia = UP ? i : i - 1;
     const int ib = UP ? ia + 3 : ia - 3;
@@ -1202,8 +1205,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     }
 #pragma unroll
     for (int k = 0; k < kFTerms; ++k) tq0[k] = rg.q[sq][k][lane];
-    const APre<T> cur = pre;
-    fetch(i + STEP, pre);
+    fetch(i + STEP, nxt);
     cu[NEW] = ct_u(cur.cr);
     cv[NEW] = cur.cr.v;
     hh[NEW] = cur.h;
@@ -1313,7 +1315,15 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     shift(cu);
     shift(cv);
     shift(hh);
+  };
+  const int cnt = a1 - a0;
+  APre<T> alt;
+  int n = 1;
+  for (; n < cnt; n += 2) {
+    step(n, pre, alt);
+    step(n + 1, alt, pre);
   }
+  if (n == cnt) step(n, pre, alt);
 }
 
 template <typename T, bool GSYM, bool SAMEP>
