@@ -136,9 +136,9 @@ int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, doubl
 /* Rows per wavefront band of the stage kernels: rows_a / rows_b for the
  * separate height and velocity kernels (default 32), rows_f for the fused
  * kernel, which runs when rows_f > 0; rows_f = 0 selects the separate
- * kernels.  sw_create sets rows_f from the resolution and the device's CU
- * count (the band height that keeps the fused waves within few rounds of
- * resident waves).  Performance only: every setting in [4, 4096] gives
+ * kernels.  sw_create sets rows_f from the resolution, the precision and the
+ * device's CU count (the band height that keeps the fused waves within few
+ * rounds of resident waves).  Performance only: every setting in [4, 4096] gives
  * bit-identical results.  A batch (sw_batch_run) uses the fused kernel when
  * every member's rows_f > 0.  Build-defined, no counterpart in the
  * reference. */

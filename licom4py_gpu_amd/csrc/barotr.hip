@@ -3118,10 +3118,11 @@ extern "C" {
 
 // Default band height of k_fused: the height that minimises (rounds of
 // resident waves) x (rows one wave marches, its band plus the 6 rows of
-// overlap), with two 4-wave blocks resident per CU.
-int default_rows_f(int nx, int cus) {
+// overlap).  Resident waves per CU: 8 in FP64 (two 4-wave blocks, 2 waves per
+// SIMD by registers), 12 in FP32 (3 per SIMD).
+int default_rows_f(int nx, int cus, int dtype) {
   const long S = (nx + kFStride - 1) / kFStride;
-  const long slots = 8L * (cus > 0 ? cus : 256);
+  const long slots = (dtype == SW_F64 ? 8L : 12L) * (cus > 0 ? cus : 256);
   int best = 32;
   long best_cost = -1;
   for (int R : {8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64}) {
@@ -3153,7 +3154,7 @@ int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CON
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 0;
-    c->rows_f = default_rows_f(nx, cus);
+    c->rows_f = default_rows_f(nx, cus, dtype);
   }
   rc = validate_consts(c, consts);
   if (!rc) rc = (dtype == SW_F64) ? do_create_planes<double>(c, consts) : do_create_planes<float>(c, consts);

@@ -249,9 +249,9 @@ def test_default_tuning_read_back():
     the device; the context reports what the library holds."""
     torch = _torch()
     from licom4py_gpu_amd import barotr
-    for nx in (48, 140):
+    for nx, dtype in ((48, torch.float64), (140, torch.float64), (140, torch.float32)):
         _, consts = grid_consts(nx)
-        ctx = barotr.SWContext(consts, torch.float64)
+        ctx = barotr.SWContext(consts, dtype)
         ra, rb, rf = ctx.tuning
         assert (ra, rb) == (32, 32) and rf in (8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64)
         ctx.set_tuning(16, 20, 0)
