@@ -1111,8 +1111,8 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   const int b0 = max(w.r0, 2 * HALO), b1 = min(w.r1, N - 2 * HALO);
   T* sink = a.sink + threadIdx.x;
 
-  // the step's inputs at face i: the newest row's column point and halo
-  // column, i-face i, cell row ia
+  // the step's inputs at face i: the newest row's column point, i-face i,
+  // cell row ia (no halo column: hl = false)
   auto fetch = [&](int i, APre<T>& p) {
     const int rn = UP ? max(i - 3, 0) : min(i + 2, N - 1);
     a_fetch<T, GSYM, SAMEP>(a, w, i, rn, UP ? min(i, N - 1) : max(i - 1, 0), rn, w.jl, p, false);
