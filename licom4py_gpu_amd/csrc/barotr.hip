@@ -727,9 +727,13 @@ __device__ __forceinline__ void b_fetch_face(const MarchArgs<T>& a, const WaveTi
   const Planes<T>& P = a.P;
   const uint32_t oi = static_cast<uint32_t>(min(i, N - 1)) * N + w.jl;
   p.cdy = ldg(P.cdy, w.mo + oi);
-  p.dzx = ldg(P.dzx, P.dz_uniform ? 0 : w.fo + oi);
-  const bool iedge = (i == HALO || i == N - HALO) && w.j <= N - 4;
-  p.evi = ldg(a.evel, iedge ? (w.t * 4 + (i == HALO ? DS : DN)) * nx + (w.j - HALO) : 0);
+  // loads no lane uses are skipped behind wave-uniform tests (as in a_fetch)
+  p.dzx = P.dz_uniform ? P.dzx_s : ldg(P.dzx, w.fo + oi);
+  p.evi = T(0);
+  if (i == HALO || i == N - HALO) {
+    const bool iedge = w.j <= N - 4;
+    p.evi = ldg(a.evel, iedge ? (w.t * 4 + (i == HALO ? DS : DN)) * nx + (w.j - HALO) : 0);
+  }
 }
 template <typename T, bool GSYM>
 __device__ __forceinline__ void b_fetch_halo(const MarchArgs<T>& a, const WaveTile& w, int ii, int hcol,
@@ -765,9 +769,12 @@ __device__ __forceinline__ void b_fetch_cell(const MarchArgs<T>& a, const WaveTi
     p.up = ldg(a.up, w.fo + om);
     p.vp = ldg(a.vp, w.fo + om);
   }
-  p.dzy = ldg(P.dzy, P.dz_uniform ? 0 : w.fo + om);
-  const bool jedge = (w.j == HALO || w.j == N - HALO) && ii >= HALO && ii < N - HALO;
-  p.evj = ldg(a.evel, jedge ? (w.t * 4 + (w.j == HALO ? DW : DE)) * nx + (ii - HALO) : 0);
+  p.dzy = P.dz_uniform ? P.dzy_s : ldg(P.dzy, w.fo + om);
+  p.evj = T(0);
+  if ((w.c0 <= HALO && HALO <= w.c0 + 63) || (w.c0 <= N - HALO && N - HALO <= w.c0 + 63)) {
+    const bool jedge = (w.j == HALO || w.j == N - HALO) && ii >= HALO && ii < N - HALO;
+    p.evj = ldg(a.evel, jedge ? (w.t * 4 + (w.j == HALO ? DW : DE)) * nx + (ii - HALO) : 0);
+  }
 }
 
 // C-grid quantities on the i-face at row i from the column windows
