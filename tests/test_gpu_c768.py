@@ -1,9 +1,10 @@
-"""The C768 workload itself (GPU): properties that hold at any size.
+"""The C768 workload itself (GPU).
 
-The oracle is too slow to step C768 here, so the full-size checks are
-size-independent: graph replay and eager launches give identical bits, the
-band height does not change a bit, the binary64 mass integral is conserved to
-round-off, and the state stays finite.
+Three substeps of the benchmark field are compared with the NumPy oracle bit
+for bit (a few seconds of oracle time).  Longer runs are checked through
+properties that hold at any size: graph replay and eager launches give
+identical bits, the band height does not change a bit, the binary64 mass
+integral is conserved to round-off, and the state stays finite.
 """
 
 from __future__ import annotations
@@ -17,17 +18,45 @@ SUBSTEPS = 8
 
 
 @pytest.fixture(scope="module")
-def c768():
+def c768_host():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    from licom4py_gpu_amd import barotr, duogrid, initial
+    from licom4py_gpu_amd import duogrid, initial
     g = duogrid.grid(NX)
     h, u, v, sp = initial.seeded_balanced(g)
     dt = initial.courant_dt(g, h, sp)
-    consts = barotr.as_device_consts(duogrid.barotr_consts(g, 5000.0), torch.device("cuda", 0))
-    core = [torch.as_tensor(x).cuda() for x in initial.state_tuple(h, u, v)[:6]]
+    return duogrid.barotr_consts(g, 5000.0), initial.state_tuple(h, u, v), dt
+
+
+@pytest.fixture(scope="module")
+def c768(c768_host):
+    import torch
+    from licom4py_gpu_amd import barotr
+    host_consts, state, dt = c768_host
+    consts = barotr.as_device_consts(host_consts, torch.device("cuda", 0))
+    core = [torch.as_tensor(x).cuda() for x in state[:6]]
     return barotr.SWContext(consts, torch.float64), core, dt
+
+
+def test_c768_three_substeps_match_oracle(c768_host):
+    """The benchmark configuration at its own size, through the drop-in
+    entry point (a context with the band height sw_create picks, recorded
+    graphs), against the oracle's bits."""
+    import numpy as np
+    import torch
+    from licom4py_gpu_amd import barotr
+    from oracle import barotr_np as B
+    host_consts, state, dt = c768_host
+    dev = torch.device("cuda", 0)
+    dconsts = barotr.as_device_consts(host_consts, dev)
+    dstate = tuple(torch.as_tensor(x).to(dev) for x in state)
+    out = barotr.barotr_rk2_core(dstate, dconsts, 3, dt)
+    got = [x.cpu().numpy() for x in out]
+    want = B.barotr_rk2_core(state, host_consts, 3, dt)
+    assert len(got) == len(want) == 23
+    for k in range(23):
+        assert np.array_equal(got[k], want[k]), k
 
 
 def test_c768_graph_equals_eager_and_band_height_is_invisible(c768):
