@@ -57,6 +57,15 @@ def test_c768_three_substeps_match_oracle(c768_host):
     assert len(got) == len(want) == 23
     for k in range(23):
         assert np.array_equal(got[k], want[k]), k
+    # the FP32 form against the same FP64 oracle: relative L2 of h, u, v
+    # within 1e-5 (the bar for FP32 heights)
+    ctx32 = barotr.SWContext(dconsts, torch.float32)
+    core32 = [torch.as_tensor(x).to(dev, torch.float32) for x in state[:6]]
+    out32 = ctx32.rk(core32, 3, dt, diagnostics=False, graph=True)
+    for k, idx in enumerate((0, 2, 3)):
+        a = out32[k].double().cpu().numpy()[:, 3:-3, 3:-3]
+        b = want[idx][:, 3:-3, 3:-3]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5, idx
 
 
 def test_c768_graph_equals_eager_and_band_height_is_invisible(c768):
