@@ -515,8 +515,8 @@ __device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRa
 // lane of the wave uses are skipped behind wave-uniform tests: the cube-edge
 // velocity and flux of the i-face off the edge rows and of the j-face in
 // strips without an edge column, the depth where it is uniform (the scalar
-// argument holds it).  hl: this lane loads the halo column (false: p.hr and
-// p.hh are left unset and the caller does not use them).
+// argument holds it).  hl: load the halo column (kernel A; the fused kernel
+// has none and passes false, leaving p.hr and p.hh unset).
 template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w, int i, int crow, int ii,
                                         int hrow, int hcol, APre<T>& p, bool hl = true) {
