@@ -81,7 +81,9 @@ int sw_set_stream(sw_ctx* ctx, void* stream);
  * With SW_SKIP_DIAG, out[3..12] are not written and may be null.
  * The reference returns h0p = h0, ubp = ub, vbp = vb and passes the coupling
  * fields through; callers alias those.  nbb = 0 copies in -> out (h0, ub, vb)
- * and leaves the diagnostics untouched, as the reference's empty loop does. */
+ * and needs SW_SKIP_DIAG: the reference's empty loop returns the input's
+ * diagnostics, which in[] does not carry, so the caller keeps its own
+ * (nbb = 0 without SW_SKIP_DIAG returns SW_E_ARG). */
 int sw_rk2(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
            double dtb, int flags);
 /* nbb RK3 substeps (barotr.py:64-87): stages dt/3, dt/2, dt with beta = 1. */
@@ -178,8 +180,9 @@ void sw_batch_destroy(sw_batch* b);
  * their own nbb RK substeps (rk = 2 or 3), every kernel class launched once
  * per stage for all members still stepping, on `stream`.  Each member's
  * outputs equal those of sw_rk2 / sw_rk3 on its context alone (flags:
- * SW_SKIP_DIAG; SW_GRAPH replays a recorded graph of substep pairs when all
- * members have the same nbb >= 3).  Generalises the reference's one-field loop
+ * SW_SKIP_DIAG; SW_GRAPH replays recorded graphs of substep pairs, one per
+ * number of members still stepping).  A member with nbb = 0 needs
+ * SW_SKIP_DIAG, as for sw_rk2.  Generalises the reference's one-field loop
  * (momentum/barotr.py:41-87) to many fields per launch.  A batch object may be
  * reused; its tables are replaced at every run. */
 int sw_batch_run(sw_batch* b, const sw_member* members, int count, int rk, int flags, void* stream);

@@ -239,27 +239,48 @@ class SWContext:
 
 
 # ---------------------------------------------------------------------------
-# context cache keyed on the identity of the consts (the reference keys its
+# context cache keyed on the buffers of the consts (the reference keys its
 # compiled graph on the static args and the arrays' shardings)
 # ---------------------------------------------------------------------------
 _CTX_CACHE: "OrderedDict[tuple, tuple]" = OrderedDict()
-_CTX_CACHE_MAX = 16
+_CTX_CACHE_MAX = 8
+_ctx_builds = 0  # contexts built by context_for (a cache miss each)
+
+
+def _buffer_key(c) -> tuple:
+    """Identity of a const's storage: a new tensor or DLPack view of the same
+    buffer (same address, shape, dtype, device, unmodified) gives the same
+    key, so a caller that rewraps its arrays on every call still hits.  Host
+    arrays carry no version counter: consts are read-only inputs (JAX arrays
+    are immutable), and one modified in place needs a new buffer."""
+    if isinstance(c, torch.Tensor):
+        return ("t", c.data_ptr(), tuple(c.shape), tuple(c.stride()), str(c.dtype), str(c.device),
+                c._version)
+    a = np.asarray(c)
+    return ("n", a.__array_interface__["data"][0], a.shape, a.strides, a.dtype.str)
 
 
 def context_for(consts: Sequence, dtype: torch.dtype) -> SWContext:
-    key = (dtype,) + tuple(
-        (id(c), c.data_ptr() if isinstance(c, torch.Tensor) else 0,
-         getattr(c, "_version", 0)) for c in consts[:N_BAROTR_CONSTS])
+    """The cached context of (consts, dtype), built on first use.  The cache
+    holds the const objects, so their buffers cannot be freed and reused while
+    an entry refers to them."""
+    global _ctx_builds
+    key = (str(dtype),) + tuple(_buffer_key(c) for c in consts[:N_BAROTR_CONSTS])
     hit = _CTX_CACHE.get(key)
     if hit is not None:
         _CTX_CACHE.move_to_end(key)
         return hit[0]
     ctx = SWContext(consts, dtype)
-    # keep the const objects alive so their ids cannot be reused while cached
+    _ctx_builds += 1
     _CTX_CACHE[key] = (ctx, tuple(consts[:N_BAROTR_CONSTS]))
     while len(_CTX_CACHE) > _CTX_CACHE_MAX:
         _CTX_CACHE.popitem(last=False)
     return ctx
+
+
+def context_builds() -> int:
+    """Number of contexts `context_for` has built in this process."""
+    return _ctx_builds
 
 
 def _check_mesh(px: int, py: int) -> None:

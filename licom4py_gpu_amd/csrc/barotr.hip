@@ -7,18 +7,27 @@
 // kernel/communication.py:355-570) become plain loads from the neighbouring
 // face's memory.
 //
-// One RK stage (momentum/barotr_pure.py:100-168) runs as five launches:
-//   k_edge         C-grid normal velocity on the 12 cube edges, averaged over
-//                  the two faces that meet there (remap.py:107 ->
-//                  communication.py:485-564); in the last stage also the
-//                  averaged mass flux (barotr_pure.py:116-117).
-//   k_march_a      new height on the interior: celerity, C-grid velocity, LMARS
-//                  correction, upwind flux, divergence (barotr_pure.py:111-120).
-//   k_halo_scalar  duo-grid halo fill of the new height (cube.py:130-134).
-//   k_march_b      new covariant velocity on the interior: D-grid upwind,
-//                  vorticity/kinetic-energy advection, LMARS dissipation,
-//                  pressure gradient, Coriolis (barotr_pure.py:133-150).
-//   k_halo_vector  vector halo fill (cube.py:137-153).
+// One RK stage (momentum/barotr_pure.py:100-168) runs as three launches:
+//   k_fused         new height on the interior (celerity, C-grid velocity,
+//                   LMARS correction, upwind flux, divergence;
+//                   barotr_pure.py:111-120) and new covariant velocity on the
+//                   deep cells, whose pressure-gradient stencil stays inside
+//                   the face (D-grid upwind, vorticity / kinetic-energy
+//                   advection, LMARS dissipation, pressure gradient, Coriolis;
+//                   barotr_pure.py:133-150), one column march for both.
+//   k_frame         velocity on the 3-cell frame along the face edges, with
+//                   the new height's duo-grid halo values formed from the halo
+//                   plan, and the halo fill of the new height itself
+//                   (cube.py:130-134) in the same launch.
+//   k_halo_uv_edge  vector halo fill of the new velocity (cube.py:137-153) and
+//                   the next stage's cube-edge pass: C-grid normal velocity
+//                   averaged over the two faces of each cube edge (remap.py:107
+//                   -> communication.py:485-564) and, for a last stage, the
+//                   averaged mass flux (barotr_pure.py:116-117).
+// The first stage of a call runs its edge pass as a launch of its own
+// (k_edge).  With the fused form switched off (band height rows_f = 0) a
+// stage runs k_march_a (height), k_halo_scalar, k_march_b (velocity on every
+// interior cell) and k_halo_uv_edge instead.
 // The ten diagnostic outputs of the last stage are written once per call by
 // k_diag1..3 over the full padded arrays, with the reference's values on the
 // halo ring.  The stage kernels march wavefronts down columns with the
@@ -2604,6 +2613,10 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
   if (!(dtb == dtb) || dtb > 1e300 || dtb < -1e300) return fail(SW_E_ARG, "dtb must be finite");
   const size_t bytes = c->cells * sizeof(T);
   if (nbb == 0) {
+    // the reference's empty loop returns the input's diagnostics, which the
+    // six input fields do not carry: refuse rather than leave them unwritten
+    if (!(flags & SW_SKIP_DIAG))
+      return fail(SW_E_ARG, "nbb == 0 computes no diagnostics: pass SW_SKIP_DIAG (the caller keeps its own)");
     const int from[3] = {0, 2, 3};
     for (int k = 0; k < 3; ++k)
       if (out[k] != in[from[k]])
@@ -2840,6 +2853,9 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     if (!rc) rc = check_disjoint(m.ctx, m.in, m.out);
     if (rc) return fail(rc, "member " + std::to_string(i) + ": " + sw_last_error());
     if (m.nbb < 0) return fail(SW_E_ARG, "member " + std::to_string(i) + ": nbb < 0");
+    if (m.nbb == 0 && !(flags & SW_SKIP_DIAG))
+      return fail(SW_E_ARG, "member " + std::to_string(i) +
+                                ": nbb == 0 computes no diagnostics: pass SW_SKIP_DIAG (the caller keeps its own)");
     if (!(m.dtb == m.dtb) || m.dtb > 1e300 || m.dtb < -1e300)
       return fail(SW_E_ARG, "member " + std::to_string(i) + ": dtb must be finite");
     for (int k = 0; k < 13; ++k)

@@ -18,7 +18,9 @@ coupling fields.
 
 from __future__ import annotations
 
+import concurrent.futures
 import math
+import os
 from typing import Dict, Tuple
 
 import numpy as np
@@ -69,13 +71,7 @@ def harmonic_coefficients(seed: int, perturb: float = 0.0, member_seed: int | No
     return terms
 
 
-def seeded_balanced(g: Dict[str, np.ndarray], seed: int = 20261203, amp: float = 200.0,
-                    perturb: float = 0.0, member_seed: int | None = None,
-                    terms=None):
-    """(h0, ub, vb, speed) on the padded grid; see module docstring."""
-    r = _unit_xyz(g["a_pt"])
-    if terms is None:
-        terms = harmonic_coefficients(seed, perturb, member_seed)
+def _harmonic_chunk(r: np.ndarray, terms):
     h = np.zeros(r.shape[:-1])
     grad = np.zeros(r.shape)
     for l, a, c in terms:
@@ -83,6 +79,42 @@ def seeded_balanced(g: Dict[str, np.ndarray], seed: int = 20261203, amp: float =
         p, dp = _legendre(l, x)
         h += c * p
         grad += (c * dp)[..., None] * (a - x[..., None] * r)
+    return h, grad
+
+
+def _harmonic_sum(r: np.ndarray, terms):
+    """Sum of the harmonic terms and of their surface gradients at the points
+    r (..., 3).  Every operation is elementwise per point, so the points are
+    split into chunks evaluated by a thread pool (NumPy releases the GIL in
+    its loops); the result does not depend on the split."""
+    flat = r.reshape(-1, 3)
+    n = flat.shape[0]
+    workers = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1))
+    nchunk = workers if n >= 65536 else 1
+    bounds = np.linspace(0, n, nchunk + 1).astype(int)
+    h = np.empty(n)
+    grad = np.empty((n, 3))
+
+    def run(k):
+        lo, hi = bounds[k], bounds[k + 1]
+        h[lo:hi], grad[lo:hi] = _harmonic_chunk(flat[lo:hi], terms)
+
+    if nchunk == 1:
+        run(0)
+    else:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=nchunk) as ex:
+            list(ex.map(run, range(nchunk)))
+    return h.reshape(r.shape[:-1]), grad.reshape(r.shape)
+
+
+def seeded_balanced(g: Dict[str, np.ndarray], seed: int = 20261203, amp: float = 200.0,
+                    perturb: float = 0.0, member_seed: int | None = None,
+                    terms=None):
+    """(h0, ub, vb, speed) on the padded grid; see module docstring."""
+    r = _unit_xyz(g["a_pt"])
+    if terms is None:
+        terms = harmonic_coefficients(seed, perturb, member_seed)
+    h, grad = _harmonic_sum(r, terms)
     interior = h[:, 3:-3, 3:-3]
     scale = amp / np.abs(interior).max()
     h *= scale

@@ -18,16 +18,33 @@ __all__ = ["make_schedule_core", "STATE_KEYS", "CONST_SOURCES",
            "N_BAROTR_STATE", "N_BAROTR_CONSTS"]
 
 
-def make_schedule_core(rk_type_barotr: int = 2, px: int = 1, py: int = 1):
-    """schedule.py:130-192: rk_type_barotr == 2 selects RK2, anything else RK3."""
+def _as_tensor(x):
+    """A torch view of a device array exported through DLPack (no copy)."""
+    import torch
+
+    return x if isinstance(x, torch.Tensor) else torch.from_dlpack(x)
+
+
+def make_schedule_core(rk_type_barotr: int = 2, px: int = 1, py: int = 1, from_dlpack=None):
+    """schedule.py:130-192: rk_type_barotr == 2 selects RK2, anything else RK3.
+
+    The returned `schedule_core` takes torch tensors.  Given `from_dlpack`
+    (the caller's array library's importer, e.g. ``jax.dlpack.from_dlpack``),
+    it also takes any device arrays that export DLPack: the state and consts
+    are viewed as tensors without a copy (the same buffers map to the same
+    cached context on every call), and every returned field is handed back
+    through `from_dlpack`, so a caller such as ``Schedule.run`` keeps its own
+    array type (its ``jax.device_get`` snapshots keep working)."""
     rk_core = barotr_rk2_core if rk_type_barotr == 2 else barotr_rk3_core
 
     def schedule_core(state, consts, chunk_size, nbb, dtb):
-        barotr_consts = tuple(consts[:N_BAROTR_CONSTS])
-        s = tuple(state)
+        barotr_consts = tuple(_as_tensor(c) for c in consts[:N_BAROTR_CONSTS])
+        s = tuple(_as_tensor(x) for x in state)
         for _ in range(int(chunk_size)):
             bs = rk_core(s[:N_BAROTR_STATE], barotr_consts, nbb, dtb, px, py)
             s = tuple(bs) + s[N_BAROTR_STATE:]
+        if from_dlpack is not None:
+            s = tuple(from_dlpack(x) for x in s)
         return s
 
     return schedule_core

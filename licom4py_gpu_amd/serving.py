@@ -4,14 +4,24 @@ A request asks for `steps` RK2 substeps of one field at resolution C{nx}, in
 FP64 or FP32, from a given initial (h, u, v).  The server
 
   * admits waiting requests in arrival order while the device bytes they will
-    hold (their workspace and two resident state buffers, plus the consts of a
-    resolution not yet resident) fit the HBM budget;
+    hold fit the HBM budget: two resident state buffers, a context workspace
+    (reused from a retired request of the same kind when one is idle) and, for
+    a kind (resolution, precision) not yet resident, its const planes;
   * runs ticks: every admitted request advances by min(chunk, remaining)
     substeps, all requests of one precision in one batched run
-    (``sw_batch_run``: one launch per kernel class and stage), the state staying
-    resident on the device between ticks;
-  * retires finished requests and admits newcomers between ticks, so the batch
-    composition changes continuously.
+    (``sw_batch_run``: one launch per kernel class and stage, recorded
+    hipGraphs of substep pairs), the state staying resident on the device
+    between ticks.  FP64 and FP32 groups go to two streams (default priority),
+    so their launches overlap on the device;
+  * never waits for the device between ticks: each group's last launch records
+    an event, the host admits, builds the next tick's tables and launches it
+    while the device still runs the previous one, and retires a request when
+    the event of its last tick has completed.  Its completion time is that
+    event's device timestamp, so latency does not include the host's polling;
+  * frees room when the request at the head of the queue does not fit: idle
+    workspaces first, then whole kinds with nothing admitted.  A request that
+    cannot fit even then (kept results hold the rest of the budget) fails with
+    an error instead of waiting forever.
 
 Each request's result is bit-identical to the same solve run alone (substeps
 are deterministic functions of the state, whatever batch they run in).
@@ -34,17 +44,20 @@ import json
 import math
 import sys
 import time
-from dataclasses import dataclass, field
-from typing import Callable, Deque, Dict, List, Optional, Sequence, Tuple
+from dataclasses import dataclass
+from typing import Callable, Deque, Dict, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 import torch
 
-from . import barotr, ensemble, initial
+from . import barotr, duogrid, ensemble, initial
 
 SCRATCH_FIELDS = 12  # stage scratch and substep ping-pong of a context workspace
-STATE_FIELDS = 6     # two resident (h, u, v) buffers per request
-CONST_PLANES = 26    # context planes of a resolution (23) + index tables + halo plan, rounded up
+STATE_FIELDS = 6     # two resident (h, u, v) buffers per admitted request
+RESULT_FIELDS = 3    # a kept result: one (h, u, v) buffer
+CONST_PLANES = 26    # context planes of a kind (23) + index tables + halo plan, rounded up
+
+Init = Union[Tuple[torch.Tensor, torch.Tensor, torch.Tensor], Callable[[], Tuple[torch.Tensor, ...]]]
 
 
 @dataclass
@@ -54,13 +67,17 @@ class Request:
     steps: int
     dtype: torch.dtype
     dt: float
-    init: Tuple[torch.Tensor, torch.Tensor, torch.Tensor]  # (h, u, v), (6, N, N), any device
+    init: Init  # (h, u, v) of shape (6, N, N) on any device, or a callable forming them at admission
     arrival: float = 0.0
     keep_result: bool = True
     t_admit: Optional[float] = None
     t_done: Optional[float] = None
     result: Optional[Tuple[torch.Tensor, ...]] = None
     checksum: Optional[float] = None
+    error: Optional[str] = None
+
+    def initial(self):
+        return self.init() if callable(self.init) else self.init
 
 
 def field_bytes(nx: int, dtype: torch.dtype) -> int:
@@ -69,65 +86,55 @@ def field_bytes(nx: int, dtype: torch.dtype) -> int:
 
 
 def request_bytes(nx: int, dtype: torch.dtype) -> int:
-    """Device bytes a request holds while admitted."""
+    """Device bytes a request holds while admitted (its own workspace)."""
     return (SCRATCH_FIELDS + STATE_FIELDS) * field_bytes(nx, dtype)
 
 
+def kind_bytes(nx: int, dtype: torch.dtype) -> int:
+    """Device bytes of a kind's const planes, held while any of its contexts lives."""
+    return CONST_PLANES * field_bytes(nx, dtype)
+
+
 class HBMLedger:
-    """Device bytes held under the budget, per kind (nx, dtype) of request:
-    admitted requests hold a workspace and two state buffers; a retired
-    request's workspace stays idle for the next request of its kind until
-    room is needed; a kind's consts stay resident once loaded."""
+    """Device bytes held under the budget."""
 
     def __init__(self, budget_bytes: float):
         self.budget = float(budget_bytes)
         self.used = 0.0
         self.peak = 0.0
-        self.resident = set()
-        self.idle: Dict[tuple, int] = collections.defaultdict(int)
 
-    def need(self, key) -> float:
-        """Bytes admitting a request of `key` adds now."""
-        if self.idle[key]:
-            return STATE_FIELDS * field_bytes(*key)
-        extra = 0 if key in self.resident else CONST_PLANES * field_bytes(*key)
-        return request_bytes(*key) + extra
+    def fits(self, nbytes: float) -> bool:
+        return self.used + nbytes <= self.budget
 
-    def fits(self, key) -> bool:
-        return self.used + self.need(key) <= self.budget
-
-    def admit(self, key) -> bool:
-        """Account an admission; True if it reuses an idle workspace."""
-        self.used += self.need(key)
+    def take(self, nbytes: float) -> None:
+        self.used += nbytes
         self.peak = max(self.peak, self.used)
-        self.resident.add(key)
-        if self.idle[key]:
-            self.idle[key] -= 1
-            return True
-        return False
 
-    def retire(self, key) -> None:
-        self.used -= STATE_FIELDS * field_bytes(*key)
-        self.idle[key] += 1
+    def give(self, nbytes: float) -> None:
+        self.used -= nbytes
 
-    def drop_idle(self):
-        """Release the largest idle workspace; returns its kind, or None."""
-        keys = [k for k, n in self.idle.items() if n]
-        if not keys:
-            return None
-        k = max(keys, key=lambda k: field_bytes(*k))
-        self.idle[k] -= 1
-        self.used -= SCRATCH_FIELDS * field_bytes(*k)
-        return k
+
+@dataclass
+class _Kind:
+    """Contexts of one kind (nx, dtype), each with a workspace.  They share the
+    kind's const planes, which stay allocated while any of them lives."""
+    live: List          # every live context; live[0] is cloned for new ones
+    idle: List          # live contexts with no request
 
 
 @dataclass
 class _Active:
     req: Request
-    ctx: "barotr.SWContext"
-    bufs: List[List[torch.Tensor]]
+    ctx: object
+    bufs: List
     cur: int = 0
     remaining: int = 0
+
+
+@dataclass
+class _Tick:
+    events: List        # one per precision group launched
+    finishing: List     # _Active entries whose last substeps this tick runs
 
 
 # Band heights of the served contexts.  A batch of many members fills the
@@ -136,22 +143,98 @@ class _Active:
 BATCH_ROWS = (32, 32, 32)
 
 
-class Server:
-    def __init__(self, budget_bytes: float = 200e9, chunk: int = 100, device=None, rows=BATCH_ROWS):
+class GPUBackend:
+    """Device side of the server: contexts, state buffers, batched runs on one
+    stream per precision, events."""
+
+    def __init__(self, device=None, rows=BATCH_ROWS, graph: bool = True):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.rows = tuple(rows) if rows is not None else None
+        self.graph = graph
+        self.streams = {torch.float64: torch.cuda.Stream(self.device), torch.float32: torch.cuda.Stream(self.device)}
+        self.batches = {torch.float64: ensemble.Batch(), torch.float32: ensemble.Batch()}
+        self._host_consts: Dict[int, tuple] = {}
+        self._ref = None
+
+    def host_consts(self, nx: int):
+        if nx not in self._host_consts:
+            self._host_consts[nx] = duogrid.barotr_consts(duogrid.grid(nx), 5000.0)
+        return self._host_consts[nx]
+
+    def new_context(self, nx: int, dtype: torch.dtype, base=None):
+        if base is not None:
+            return base.like()
+        # from host consts: the device copy sw_create reads is temporary
+        ctx = barotr.SWContext(self.host_consts(nx), dtype)
+        if self.rows is not None:
+            ctx.set_tuning(*self.rows)
+        return ctx
+
+    def free_context(self, ctx) -> None:
+        del ctx  # sw_destroy when the last reference goes (waits for its stream)
+
+    def admit(self, ctx, dtype: torch.dtype, init):
+        """Two state buffers; the first receives `init` on the precision's stream."""
+        s = self.streams[dtype]
+        s.wait_stream(torch.cuda.current_stream(self.device))  # the caller's init tensors are ready
+        with torch.cuda.stream(s):
+            bufs = [ctx.empty(3), ctx.empty(3)]
+            x = init() if callable(init) else init
+            for k in range(3):
+                bufs[0][k].copy_(x[k].to(device=self.device, dtype=dtype), non_blocking=True)
+        return bufs
+
+    def run(self, dtype: torch.dtype, items):
+        """items: (ctx, (h, u, v) in, (h, u, v) out, substeps, dt); returns an event."""
+        s = self.streams[dtype]
+        with torch.cuda.stream(s):
+            members = [ensemble.Member(ctx, (x[0], x[0], x[1], x[2], x[1], x[2]), n, dt, out=list(y))
+                       for ctx, x, y, n, dt in items]
+            self.batches[dtype].run(members, diagnostics=False, graph=self.graph)
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(s)
+        return ev
+
+    def start(self) -> None:
+        """Reference point of event times: the device is idle when it is taken."""
+        torch.cuda.synchronize(self.device)
+        self._ref = torch.cuda.Event(enable_timing=True)
+        self._ref.record(torch.cuda.current_stream(self.device))
+        self._ref.synchronize()
+
+    def done(self, ev) -> bool:
+        return ev.query()
+
+    def wait(self, ev) -> None:
+        ev.synchronize()
+
+    def seconds(self, ev) -> float:
+        """Device time of `ev` since start()."""
+        return self._ref.elapsed_time(ev) / 1e3
+
+    def finish(self) -> None:
+        torch.cuda.synchronize(self.device)
+
+
+class Server:
+    def __init__(self, budget_bytes: float = 200e9, chunk: int = 100, device=None, rows=BATCH_ROWS,
+                 backend=None, depth: int = 2, graph: bool = True):
+        self.backend = backend or GPUBackend(device, rows, graph)
         self.budget = float(budget_bytes)
         self.chunk = int(chunk)
-        self.pool = ensemble.GridPool(self.device, rows=rows)
-        self.batches = {torch.float64: ensemble.Batch(), torch.float32: ensemble.Batch()}
+        self.depth = max(1, int(depth))  # ticks the host may have queued on the device
         self.queue: Deque[Request] = collections.deque()
         self.active: List[_Active] = []
+        self.inflight: Deque[_Tick] = collections.deque()
         self.done: List[Request] = []
+        self.failed: List[Request] = []
         self.ledger = HBMLedger(budget_bytes)
-        # retired requests' contexts, reused by later requests of the same kind
-        self._idle: Dict[Tuple[int, torch.dtype], List["barotr.SWContext"]] = collections.defaultdict(list)
+        self.kinds: Dict[Tuple[int, torch.dtype], _Kind] = {}
         self.ticks = 0
         self.batch_sizes: List[int] = []
         self.cell_substeps = 0
+        self.evictions = 0
+        self._t0_host = 0.0
 
     # -- admission ------------------------------------------------------------
     def submit(self, req: Request) -> None:
@@ -159,8 +242,7 @@ class Server:
             raise ValueError("steps must be >= 0")
         if req.dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be float64 or float32")
-        need = request_bytes(req.nx, req.dtype) + CONST_PLANES * field_bytes(req.nx, req.dtype)
-        if need > self.budget:
+        if request_bytes(req.nx, req.dtype) + kind_bytes(req.nx, req.dtype) > self.budget:
             raise ValueError(f"request {req.rid} (C{req.nx}) does not fit the HBM budget")
         self.queue.append(req)
 
@@ -172,85 +254,173 @@ class Server:
     def peak_used(self) -> float:
         return self.ledger.peak
 
+    def _need(self, key) -> float:
+        fb = field_bytes(*key)
+        k = self.kinds.get(key)
+        if k is None:
+            return kind_bytes(*key) + (SCRATCH_FIELDS + STATE_FIELDS) * fb
+        if k.idle:
+            return STATE_FIELDS * fb
+        return (SCRATCH_FIELDS + STATE_FIELDS) * fb
+
+    def _drop_context(self, key) -> None:
+        """Free one idle context of `key`; the kind's planes go with its last."""
+        k = self.kinds[key]
+        ctx = k.idle.pop()
+        k.live = [c for c in k.live if c is not ctx]
+        self.ledger.give(SCRATCH_FIELDS * field_bytes(*key))
+        if not k.live:
+            self.ledger.give(kind_bytes(*key))
+            del self.kinds[key]
+        self.backend.free_context(ctx)
+        self.evictions += 1
+
+    def _make_room(self, key, need: float) -> bool:
+        """Free idle contexts until `need` bytes fit: the largest workspaces
+        first, whole kinds (planes too) once their last context goes; the
+        request's own kind keeps the idle context it will reuse."""
+        while not self.ledger.fits(need):
+            cands = [q for q, k in self.kinds.items() if len(k.idle) > (1 if q == key else 0)]
+            if not cands:
+                return False
+            self._drop_context(max(cands, key=lambda q: field_bytes(*q)))
+        return True
+
     def _admit(self, now: float) -> None:
         while self.queue:
             req = self.queue[0]
             key = (req.nx, req.dtype)
-            if not self.ledger.fits(key):
-                dropped = self.ledger.drop_idle()
-                if dropped is None:
-                    break
-                self._idle[dropped].pop()  # its workspace is freed with the context
+            need = self._need(key)
+            if not self.ledger.fits(need) and not self._make_room(key, need):
+                if self.active or self.inflight:
+                    break  # room comes back as admitted requests finish
+                # nothing runs and nothing idle is left to free: it never fits
+                self.queue.popleft()
+                req.error = (f"request {req.rid} (C{req.nx}, {req.dtype}) needs {need / 1e9:.3f} GB with "
+                             f"{self.ledger.used / 1e9:.3f} GB of {self.budget / 1e9:.3f} GB held by kept results "
+                             "and resident kinds")
+                self.failed.append(req)
                 continue
             self.queue.popleft()
-            reuse = self.ledger.admit(key)
-            ctx = self._idle[key].pop() if reuse else self.pool.context(req.nx, req.dtype)
-            bufs = [ctx.empty(3), ctx.empty(3)]
-            for k in range(3):
-                bufs[0][k].copy_(req.init[k].to(device=self.device, dtype=req.dtype))
+            self.ledger.take(need)
+            k = self.kinds.get(key)
+            if k is None:
+                ctx = self.backend.new_context(req.nx, req.dtype)
+                self.kinds[key] = _Kind(live=[ctx], idle=[])
+            elif k.idle:
+                ctx = k.idle.pop()
+            else:
+                ctx = self.backend.new_context(req.nx, req.dtype, base=k.live[0])
+                k.live.append(ctx)
+            bufs = self.backend.admit(ctx, req.dtype, req.init)
             req.t_admit = now
-            self.active.append(_Active(req, ctx, bufs, 0, req.steps))
+            a = _Active(req, ctx, bufs, 0, req.steps)
+            if req.steps == 0:
+                self.inflight.append(_Tick([], [a]))  # nothing to run; retired with the next poll
+            else:
+                self.active.append(a)
 
-    # -- one tick ---------------------------------------------------------------
-    def _retire(self, a: _Active, now: float) -> None:
+    # -- retirement -------------------------------------------------------------
+    def _retire(self, a: _Active, t_done: float) -> None:
+        key = (a.req.nx, a.req.dtype)
+        fb = field_bytes(*key)
         h, u, v = a.bufs[a.cur]
-        a.req.t_done = now
+        a.req.t_done = t_done
         a.req.checksum = float(h[:, 3:-3, 3:-3].double().sum())
-        a.req.result = (h, u, v) if a.req.keep_result else None
-        self.ledger.retire((a.req.nx, a.req.dtype))  # the workspace stays, idle
-        self._idle[(a.req.nx, a.req.dtype)].append(a.ctx)
+        self.ledger.give(STATE_FIELDS * fb)
+        if a.req.keep_result:
+            a.req.result = (h, u, v)
+            self.ledger.take(RESULT_FIELDS * fb)
+        self.kinds[key].idle.append(a.ctx)
         self.done.append(a.req)
 
-    def tick(self, clock: Callable[[], float]) -> None:
-        """Admit what fits, advance every admitted request by up to `chunk`
-        substeps, retire the finished ones."""
+    def release(self, req: Request) -> None:
+        """Drop a kept result and its bytes from the budget."""
+        if req.result is not None:
+            req.result = None
+            self.ledger.give(RESULT_FIELDS * field_bytes(req.nx, req.dtype))
+
+    def _complete(self, t: _Tick) -> None:
+        when = max((self.backend.seconds(e) for e in t.events), default=None)
+        for a in t.finishing:
+            self._retire(a, when if when is not None else self._host_now())
+        self.inflight.remove(t)
+
+    def _poll(self) -> None:
+        """Retire the requests of every tick the device has finished."""
+        for t in list(self.inflight):
+            if all(self.backend.done(e) for e in t.events):
+                self._complete(t)
+            else:
+                break  # ticks finish in order
+
+    def _wait_oldest(self) -> None:
+        t = self.inflight[0]
+        for e in t.events:
+            self.backend.wait(e)
+        self._complete(t)
+
+    # -- one tick ---------------------------------------------------------------
+    def tick(self, clock: Callable[[], float]) -> bool:
+        """Retire finished work, admit what fits, launch every admitted
+        request's next `chunk` substeps (one batched run per precision) and
+        return without waiting for them; True if anything was launched."""
+        self._poll()
         self._admit(clock())
         if not self.active:
-            return
-        for dt_, batch in self.batches.items():
-            group = [a for a in self.active if a.req.dtype == dt_ and a.remaining > 0]
+            return False
+        events, finishing = [], []
+        for dt_ in (torch.float64, torch.float32):
+            group = [a for a in self.active if a.req.dtype == dt_]
             if not group:
                 continue
-            members = []
+            items = []
             for a in group:
-                h, u, v = a.bufs[a.cur]
                 n = min(self.chunk, a.remaining)
-                members.append(ensemble.Member(a.ctx, (h, h, u, v, u, v), n, a.req.dt,
-                                               out=a.bufs[1 - a.cur]))
+                items.append((a.ctx, a.bufs[a.cur], a.bufs[1 - a.cur], n, a.req.dt))
                 self.cell_substeps += 6 * a.req.nx * a.req.nx * n
-            batch.run(members, diagnostics=False)
-            for a, m in zip(group, members):
+            events.append(self.backend.run(dt_, items))
+            for a, it in zip(group, items):
                 a.cur = 1 - a.cur
-                a.remaining -= m.nbb
+                a.remaining -= it[3]
             self.batch_sizes.append(len(group))
-        torch.cuda.synchronize(self.device)
-        now = clock()
+        finishing = [a for a in self.active if a.remaining <= 0]
+        self.active = [a for a in self.active if a.remaining > 0]
+        self.inflight.append(_Tick(events, finishing))
         self.ticks += 1
-        still = []
-        for a in self.active:
-            if a.remaining <= 0:
-                self._retire(a, now)
-            else:
-                still.append(a)
-        self.active = still
+        while len(self.inflight) > self.depth:
+            self._wait_oldest()
+        return True
+
+    def _host_now(self) -> float:
+        return time.perf_counter() - self._t0_host
 
     def run_trace(self, requests: Sequence[Request], clock: Callable[[], float] = time.perf_counter):
         """Replay `requests` (arrival times in seconds from the start) in real
-        time; returns when all have completed."""
+        time; returns the completed requests when all have completed or
+        failed (``failed`` lists those that could never fit)."""
         pending = sorted(requests, key=lambda r: r.arrival)
+        self.backend.start()
         t0 = clock()
+        self._t0_host = time.perf_counter()
         rel = lambda: clock() - t0  # noqa: E731
         i = 0
-        while i < len(pending) or self.queue or self.active:
+        while i < len(pending) or self.queue or self.active or self.inflight:
             now = rel()
             while i < len(pending) and pending[i].arrival <= now:
                 self.submit(pending[i])
                 i += 1
-            if not self.active and not self.queue:
-                if i < len(pending):
-                    time.sleep(max(0.0, min(pending[i].arrival - rel(), 0.01)))
+            if self.queue or self.active:
+                if self.tick(rel):
+                    continue
+            if self.inflight:
+                self._wait_oldest()  # nothing could be launched: room or results come from finished work
                 continue
-            self.tick(rel)
+            if self.queue:
+                continue  # the next admission pass frees room or fails the head request
+            if i < len(pending):
+                time.sleep(max(0.0, min(pending[i].arrival - rel(), 0.01)))
+        self.backend.finish()
         return self.done
 
 
@@ -286,7 +456,7 @@ def one_at_a_time(requests: Sequence[Request], pool: "ensemble.GridPool", force_
         key = (r.nx, dtype)
         if key not in ctxs:
             ctxs[key] = pool.context(r.nx, dtype)
-        h, u, v = (x.to(device=pool.device, dtype=dtype) for x in r.init)
+        h, u, v = (x.to(device=pool.device, dtype=dtype) for x in r.initial())
         torch.cuda.synchronize(pool.device)
         t0 = time.perf_counter()
         ctxs[key].rk((h, h, u, v, u, v), r.steps, r.dt, diagnostics=False, graph=True)
@@ -303,7 +473,10 @@ def seeded_trace(n: int, rate: float, resolutions: Sequence[int], steps_range=(5
                  fp32_share: float = 0.5, seed: int = 7, device=None, keep_result: bool = False):
     """Poisson arrivals of the seeded balanced recipe: each request scales one
     per-resolution base field (h, u, v) by 1 + 0.01 z (geostrophic balance is
-    linear, so the scaled field stays balanced) with a Courant dt."""
+    linear, so the scaled field stays balanced) with a Courant dt.  The scaled
+    field is formed on the device at admission, so a queued request holds no
+    device memory; the base fields (three FP64 fields per resolution) belong
+    to the trace."""
     device = device or torch.device("cuda", torch.cuda.current_device())
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n))
@@ -313,21 +486,21 @@ def seeded_trace(n: int, rate: float, resolutions: Sequence[int], steps_range=(5
     fp32 = rng.uniform(size=n) < fp32_share
     scale = 1.0 + 0.01 * rng.normal(size=n)
     base = {}
-    pool = ensemble.GridPool(device)
     out = []
     for k in range(n):
         nx = int(nxs[k])
         if nx not in base:
-            g, _ = pool.consts(nx)
+            g = duogrid.grid(nx)
             h, u, v, sp = initial.seeded_balanced(g)
             dt = initial.courant_dt(g, h, sp)
             base[nx] = ([torch.as_tensor(x).to(device) for x in (h, u, v)], dt)
-        (h, u, v), dt = base[nx]
+        fields, dt = base[nx]
         f = float(scale[k])
-        init = (h * f, u * f, v * f)
+        init = (lambda fields=fields, f=f: tuple(x * f for x in fields))
         out.append(Request(rid=k, nx=nx, steps=int(steps[k]),
                            dtype=torch.float32 if fp32[k] else torch.float64,
                            dt=dt / max(f, 1.0), init=init, arrival=float(t[k]), keep_result=keep_result))
+    torch.cuda.synchronize(device)
     return out
 
 
@@ -358,8 +531,8 @@ def serve_report(requests: int, rate: float, res: Sequence[int], steps_range, bu
     throughput; `baselines` adds the plain path one request at a time with
     each request's own precision ("mixed") and all in FP64 ("fp64")."""
     reqs = seeded_trace(requests, rate, res, steps_range, seed=seed, device=dev)
-    # make each resolution's consts resident and load the kernels before the
-    # timed trace (a warm service), then reset the statistics
+    # make each kind resident and load the kernels before the timed trace (a
+    # warm service), then reset the statistics
     srv = Server(budget_gb * 1e9, chunk, dev)
     warm = []
     for dt_ in (torch.float64, torch.float32):
@@ -383,11 +556,12 @@ def serve_report(requests: int, rate: float, res: Sequence[int], steps_range, bu
         "mean_batch": float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0,
         "max_batch": int(max(srv.batch_sizes)) if srv.batch_sizes else 0,
         "peak_hbm_gb": srv.peak_used / 1e9, "budget_gb": budget_gb,
-        "finite": bool(all(math.isfinite(r.checksum) for r in done)),
+        "failed": len(srv.failed),
+        "finite": bool(all(math.isfinite(r.checksum) for r in done)) and not srv.failed,
     }
     # the plain path runs each request in a context of the library's default
     # band heights (chosen for one context alone on the device)
-    plain = srv.pool.sibling()
+    plain = ensemble.GridPool(dev)
     if "mixed" in baselines:
         line["one_at_a_time"] = one_at_a_time(reqs, plain)
     if "fp64" in baselines:

@@ -1,5 +1,8 @@
-"""Host-side logic of the serving loop (CPU): HBM accounting and the FCFS
-comparison model."""
+"""Host-side logic of the serving loop (CPU): HBM accounting, admission,
+eviction, failure of requests that can never fit, retirement from events, and
+the FCFS comparison model.  A stub backend stands in for the device: a
+"substep" adds 1 to every value of the state, and events complete when the
+test says so."""
 
 from __future__ import annotations
 
@@ -8,60 +11,191 @@ import pytest
 import torch
 
 from licom4py_gpu_amd import serving
-from licom4py_gpu_amd.serving import CONST_PLANES, SCRATCH_FIELDS, STATE_FIELDS, HBMLedger, field_bytes
+from licom4py_gpu_amd.serving import (CONST_PLANES, RESULT_FIELDS, SCRATCH_FIELDS, STATE_FIELDS, HBMLedger,
+                                      field_bytes, kind_bytes, request_bytes)
+
+
+class _Event:
+    def __init__(self, backend):
+        self.backend = backend
+        self.t = None  # device time once complete
+
+    def ready(self):
+        return self.t is not None
+
+
+class StubBackend:
+    """Contexts are tokens, buffers are small CPU tensors, a batched run adds
+    the substep count to the state; events complete immediately unless
+    `hold` is set (then on wait / release_all)."""
+
+    def __init__(self, hold=False):
+        self.hold = hold
+        self.created = []   # (nx, dtype, cloned)
+        self.freed = 0
+        self.pending = []
+        self.clock = 0.0
+        self.runs = []
+
+    def new_context(self, nx, dtype, base=None):
+        ctx = {"nx": nx, "dtype": dtype, "id": len(self.created)}
+        self.created.append((nx, dtype, base is not None))
+        return ctx
+
+    def free_context(self, ctx):
+        self.freed += 1
+
+    def admit(self, ctx, dtype, init):
+        x = init() if callable(init) else init
+        n = ctx["nx"] + 6
+        bufs = [[torch.zeros((6, n, n), dtype=dtype) for _ in range(3)] for _ in range(2)]
+        for k in range(3):
+            bufs[0][k].copy_(x[k].to(dtype))
+        return bufs
+
+    def run(self, dtype, items):
+        self.runs.append(len(items))
+        for ctx, x, y, n, dt in items:
+            for k in range(3):
+                y[k].copy_(x[k] + n)
+        ev = _Event(self)
+        self.clock += 1.0
+        if self.hold:
+            self.pending.append(ev)
+        else:
+            ev.t = self.clock
+        return ev
+
+    def start(self):
+        pass
+
+    def done(self, ev):
+        return ev.ready()
+
+    def wait(self, ev):
+        if ev.t is None:
+            ev.t = self.clock
+            self.pending.remove(ev)
+
+    def seconds(self, ev):
+        return ev.t
+
+    def finish(self):
+        for ev in list(self.pending):
+            self.wait(ev)
+
+
+def _req(rid, nx, steps, dtype=torch.float64, keep=True, arrival=0.0, val=0.0):
+    n = nx + 6
+    init = tuple(torch.full((6, n, n), val + k, dtype=torch.float64) for k in range(3))
+    return serving.Request(rid, nx, steps, dtype, 1.0, init, arrival=arrival, keep_result=keep)
 
 
 def test_field_bytes():
     assert field_bytes(48, torch.float64) == 6 * 54 * 54 * 8
     assert field_bytes(48, torch.float32) == 6 * 54 * 54 * 4
+    assert request_bytes(48, torch.float64) == (SCRATCH_FIELDS + STATE_FIELDS) * field_bytes(48, torch.float64)
+    assert kind_bytes(48, torch.float32) == CONST_PLANES * field_bytes(48, torch.float32)
 
 
-def test_ledger_first_admission_pays_consts_and_workspace():
-    k = (48, torch.float64)
-    L = HBMLedger(1e12)
-    f = field_bytes(*k)
-    assert L.need(k) == (SCRATCH_FIELDS + STATE_FIELDS + CONST_PLANES) * f
-    assert L.admit(k) is False
-    assert L.used == (SCRATCH_FIELDS + STATE_FIELDS + CONST_PLANES) * f
-    # the consts are resident now
-    assert L.need(k) == (SCRATCH_FIELDS + STATE_FIELDS) * f
+def test_ledger():
+    L = HBMLedger(100.0)
+    assert L.fits(100.0) and not L.fits(100.5)
+    L.take(60.0)
+    L.take(30.0)
+    L.give(50.0)
+    assert L.used == 40.0 and L.peak == 90.0
 
 
-def test_ledger_retire_keeps_workspace_idle_and_reuses_it():
-    k = (96, torch.float32)
-    f = field_bytes(*k)
-    L = HBMLedger(1e12)
-    L.admit(k)
-    L.retire(k)
-    assert L.used == (SCRATCH_FIELDS + CONST_PLANES) * f  # state freed, workspace idle
-    assert L.need(k) == STATE_FIELDS * f
-    assert L.admit(k) is True                               # reuses the idle workspace
-    assert L.used == (SCRATCH_FIELDS + STATE_FIELDS + CONST_PLANES) * f
-    assert L.idle[k] == 0
+def test_results_are_the_substeps_and_bytes_return():
+    b = StubBackend()
+    srv = serving.Server(1e12, chunk=4, backend=b)
+    reqs = [_req(0, 8, 10), _req(1, 8, 3), _req(2, 12, 0), _req(3, 12, 7, torch.float32)]
+    done = srv.run_trace(reqs)
+    assert sorted(r.rid for r in done) == [0, 1, 2, 3] and not srv.failed
+    for r in done:
+        for k in range(3):
+            assert torch.all(r.result[k][:, 3:-3, 3:-3] == k + r.steps)
+    # chunks of 4: request 0 runs 4 + 4 + 2 substeps, request 1 one tick of 3
+    assert srv.ticks == 3
+    # what stays held: the kinds' planes and workspaces, and the kept results
+    fb8, fb12, fb12s = field_bytes(8, torch.float64), field_bytes(12, torch.float64), field_bytes(12, torch.float32)
+    held = (kind_bytes(8, torch.float64) + kind_bytes(12, torch.float64) + kind_bytes(12, torch.float32)
+            + 2 * SCRATCH_FIELDS * fb8 + SCRATCH_FIELDS * (fb12 + fb12s)
+            + RESULT_FIELDS * (2 * fb8 + fb12 + fb12s))
+    assert srv.used == pytest.approx(held)
+    for r in done:
+        srv.release(r)
+    assert srv.used == pytest.approx(held - RESULT_FIELDS * (2 * fb8 + fb12 + fb12s))
 
 
-def test_ledger_budget_and_drop_idle():
-    a, b = (48, torch.float64), (96, torch.float64)
-    fa, fb = field_bytes(*a), field_bytes(*b)
-    budget = (SCRATCH_FIELDS + STATE_FIELDS + CONST_PLANES) * (fa + fb)
-    L = HBMLedger(budget)
-    assert L.fits(a) and L.admit(a) is False
-    assert L.fits(b)
-    L.admit(b)
-    assert not L.fits(b)               # a second C96 does not fit
-    L.retire(a)
-    assert L.drop_idle() == a          # the idle C48 workspace is released
-    assert L.drop_idle() is None
-    assert L.used == (SCRATCH_FIELDS + STATE_FIELDS + CONST_PLANES) * fb + CONST_PLANES * fa
-    assert L.peak == budget
+def test_idle_workspace_is_reused():
+    b = StubBackend()
+    srv = serving.Server(1e12, chunk=8, backend=b)
+    srv.run_trace([_req(0, 8, 2), _req(1, 8, 2, arrival=0.0)])
+    srv.run_trace([_req(2, 8, 2)])
+    # two contexts of C8 (the second cloned), the third request reuses one
+    assert b.created == [(8, torch.float64, False), (8, torch.float64, True)]
 
 
-def test_drop_idle_takes_largest_first():
-    L = HBMLedger(1e12)
-    for k in ((48, torch.float64), (384, torch.float32), (96, torch.float64)):
-        L.admit(k)
-        L.retire(k)
-    assert L.drop_idle() == (384, torch.float32)
+def test_head_request_evicts_unused_kinds_instead_of_stalling():
+    """The stall the ledger used to allow: consts of other kinds resident,
+    nothing admitted and nothing idle of the head request's kind.  The server
+    must free the unused kinds and admit it."""
+    b = StubBackend()
+    big = (48, torch.float64)
+    fb = field_bytes(*big)
+    budget = kind_bytes(*big) + request_bytes(*big)  # exactly one C48 FP64 kind and request
+    srv = serving.Server(budget, chunk=4, backend=b)
+    srv.run_trace([_req(0, 16, 2, keep=False), _req(1, 8, 2, torch.float32, keep=False)])
+    assert srv.used > 0 and len(srv.kinds) == 2     # two small kinds resident and idle
+    assert not srv.ledger.fits(kind_bytes(*big) + request_bytes(*big))
+    done = srv.run_trace([_req(2, 48, 3, keep=False)])
+    assert [r.rid for r in done[-1:]] == [2] and not srv.failed
+    assert srv.evictions == 2 and list(srv.kinds) == [big]
+    assert srv.used == pytest.approx(kind_bytes(*big) + SCRATCH_FIELDS * fb)
+    assert srv.peak_used <= budget
+
+
+def test_request_that_can_never_fit_fails_instead_of_looping():
+    """Kept results hold most of the budget: with nothing running and nothing
+    idle to free, the head request fails with an error and the trace ends."""
+    b = StubBackend()
+    k = (16, torch.float64)
+    fb = field_bytes(*k)
+    budget = kind_bytes(*k) + request_bytes(*k) + 2 * RESULT_FIELDS * fb
+    srv = serving.Server(budget, chunk=4, backend=b)
+    first = srv.run_trace([_req(0, 16, 1), _req(1, 16, 1, arrival=0.0)])
+    assert len(first) == 2  # two kept results
+    # the kind and its workspaces can be freed, but the results stay: a C16
+    # request needs kind + workspace + state, more than what is left
+    done = srv.run_trace([_req(2, 16, 1), _req(3, 16, 1)])
+    assert [r.rid for r in done[2:]] == [2]
+    assert [r.rid for r in srv.failed] == [3] and "GB" in srv.failed[0].error
+    for r in list(srv.done):
+        srv.release(r)
+    srv.failed.clear()
+    srv.run_trace([_req(4, 16, 1)])
+    assert srv.done[-1].rid == 4 and not srv.failed
+
+
+def test_ticks_run_ahead_of_retirement():
+    """The host launches the next tick before the device finishes the last
+    one (up to `depth` ticks queued); a request is retired, and its bytes
+    returned, only when the event of its last tick has completed, with that
+    event's time as its completion time."""
+    b = StubBackend(hold=True)
+    srv = serving.Server(1e12, chunk=2, backend=b, depth=2)
+    srv.submit(_req(0, 8, 2))
+    srv.submit(_req(1, 8, 6))
+    clock = iter(np.arange(0.0, 100.0, 0.5))
+    assert srv.tick(lambda: next(clock))          # tick 1: both requests; request 0 finishes here
+    assert len(srv.inflight) == 1 and srv.done == []
+    assert srv.tick(lambda: next(clock))          # tick 2 queued behind tick 1, nothing waited for
+    assert len(srv.inflight) == 2 and srv.done == [] and b.pending
+    srv.tick(lambda: next(clock))                 # tick 3: over the depth, tick 1 is waited for
+    assert [r.rid for r in srv.done] == [0]
+    assert srv.done[0].t_done == 3.0              # the event's time (stub clock at the wait)
 
 
 def test_fcfs_latencies():
@@ -88,3 +222,15 @@ def test_global_diag_and_format():
     text = format_diag("Final time", st)
     assert text.splitlines()[0] == "Final time"
     assert "Global h0:" in text and "47.5000000000000000" in text
+
+
+def test_surplus_idle_workspaces_of_the_same_kind_are_freed():
+    b = StubBackend()
+    k = (16, torch.float64)
+    fb = field_bytes(*k)
+    srv = serving.Server(1e12, chunk=4, backend=b)
+    srv.run_trace([_req(i, 16, 1, keep=False) for i in range(3)])  # three contexts, all idle now
+    assert len(srv.kinds[k].idle) == 3
+    srv.ledger.budget = srv.used + STATE_FIELDS * fb - 1.0  # one byte short of admitting into an idle one
+    srv.run_trace([_req(9, 16, 1, keep=False)])
+    assert srv.done[-1].rid == 9 and not srv.failed and srv.evictions == 1
