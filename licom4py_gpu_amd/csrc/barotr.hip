@@ -2241,13 +2241,15 @@ struct sw_ctx {
 };
 
 // Reusable state of sw_batch_run: the device copy of the member tables and
-// its pinned host staging buffer.
+// two pinned host staging buffers used in turn, so a run never waits for the
+// previous run's staging copy (only for the one before it).
 struct sw_batch {
   void* dev = nullptr;
   size_t dev_cap = 0;
-  void* host = nullptr;
-  size_t host_cap = 0;
-  hipEvent_t staged = nullptr;   // recorded after the staging copy of the last run
+  void* host[2] = {nullptr, nullptr};
+  size_t host_cap[2] = {0, 0};
+  hipEvent_t staged[2] = {nullptr, nullptr};  // recorded after each buffer's last staging copy
+  int slot = 0;                  // staging buffer of the next run
   hipStream_t stream = nullptr;  // stream of the last run
   bool used = false;
   hipStream_t capture = nullptr;  // private stream used only to record graphs
@@ -2256,6 +2258,24 @@ struct sw_batch {
     hipGraphExec_t exec;
   };
   std::vector<Graph> graphs;     // recorded substep pairs
+  struct Retired {
+    hipGraphExec_t exec;
+    hipEvent_t done;  // recorded when the graph left the cache
+  };
+  std::vector<Retired> retired;  // evicted graphs, destroyed once `done` completes
+  void reap(bool wait) {
+    std::vector<Retired> keep;
+    for (auto& r : retired) {
+      if (wait) (void)hipEventSynchronize(r.done);
+      if (hipEventQuery(r.done) == hipSuccess) {
+        (void)hipGraphExecDestroy(r.exec);
+        (void)hipEventDestroy(r.done);
+      } else {
+        keep.push_back(r);
+      }
+    }
+    retired.swap(keep);
+  }
 };
 
 namespace {
@@ -2829,6 +2849,7 @@ int run_integrals(sw_ctx* c, const void* h, const void* u, const void* v, double
 // batched runs
 // ---------------------------------------------------------------------------
 constexpr int kBatchMax = 4096;
+constexpr size_t kBatchGraphs = 16;  // recorded substep-pair graphs kept per batch object
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
@@ -2967,13 +2988,17 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
   }
   const size_t bytes = off;
 
-  // ---- stage into pinned memory and copy (the previous run's copy must be done)
-  if (b->staged) SW_HIP(hipEventSynchronize(b->staged));
-  if (b->host_cap < bytes) {
-    if (b->host) SW_HIP(hipHostFree(b->host));
-    b->host = nullptr;
-    SW_HIP(hipHostMalloc(&b->host, bytes, hipHostMallocDefault));
-    b->host_cap = bytes;
+  b->reap(false);
+  // ---- stage into pinned memory and copy (the copy that last read this
+  // staging buffer, two runs ago, must be done)
+  const int sl = b->slot;
+  b->slot ^= 1;
+  if (b->staged[sl]) SW_HIP(hipEventSynchronize(b->staged[sl]));
+  if (b->host_cap[sl] < bytes) {
+    if (b->host[sl]) SW_HIP(hipHostFree(b->host[sl]));
+    b->host[sl] = nullptr;
+    SW_HIP(hipHostMalloc(&b->host[sl], bytes, hipHostMallocDefault));
+    b->host_cap[sl] = bytes;
   }
   if (b->dev_cap < bytes) {
     if (b->used && b->stream) SW_HIP(hipStreamSynchronize(b->stream));  // kernels may still read it
@@ -2986,7 +3011,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     // the previous run's launches on another stream still read the tables
     SW_HIP(hipStreamSynchronize(b->stream));
   }
-  char* h = static_cast<char*>(b->host);
+  char* h = static_cast<char*>(b->host[sl]);
   std::memcpy(h + o_tab, tab.data(), ntab * sizeof(StageArgs<T>));
   std::memcpy(h + o_nbb, nbb.data(), K * sizeof(int));
   for (int k = 0; k < BK_CLASSES; ++k) {
@@ -2995,9 +3020,9 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     for (int q = 0; q < K; ++q)
       for (int x = first[k][q]; x < first[k][q + 1]; ++x) blk[x] = q;
   }
-  SW_HIP(hipMemcpyAsync(b->dev, b->host, bytes, hipMemcpyHostToDevice, stream));
-  if (!b->staged) SW_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
-  SW_HIP(hipEventRecord(b->staged, stream));
+  SW_HIP(hipMemcpyAsync(b->dev, b->host[sl], bytes, hipMemcpyHostToDevice, stream));
+  if (!b->staged[sl]) SW_HIP(hipEventCreateWithFlags(&b->staged[sl], hipEventDisableTiming));
+  SW_HIP(hipEventRecord(b->staged[sl], stream));
   b->stream = stream;
   b->used = true;
 
@@ -3070,46 +3095,72 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     return SW_OK;
   };
   const int nmax = nbb[0];
-  if ((flags & SW_GRAPH) && nbb[K - 1] == nmax && nmax >= 3) {
-    // every member steps nmax substeps: substep 0 eagerly, then recorded
-    // (odd, even) substep pairs; the graph reads the tables from b->dev, so it
-    // stays valid for any members with the same table layout and grids
+  if ((flags & SW_GRAPH) && nmax >= 3) {
+    // Substep 0 eagerly, then segments in which the members still stepping
+    // (a prefix of the table, members sorted by decreasing nbb) stay the
+    // same: each segment replays a recorded (odd, even) substep pair for its
+    // prefix length, with an eager substep to reach an odd start or finish an
+    // odd count.  A graph reads the tables from b->dev, so it stays valid for
+    // any later run with the same table layout, grids and prefix.
     int rc = substep(0, K, stream);
     if (rc) return rc;
-    std::vector<long long> sig = {reinterpret_cast<long long>(b->dev), static_cast<long long>(o_tab),
-                                  static_cast<long long>(o_nbb), K, rk, gsym ? 1 : 0, fused ? 1 : 0,
-                                  static_cast<long long>(sizeof(T))};
-    for (int k = 0; k < BK_CLASSES; ++k) {
-      sig.push_back(static_cast<long long>(o_first[k]));
-      sig.push_back(static_cast<long long>(o_blk[k]));
-      sig.push_back(first[k][K]);
-    }
-    hipGraphExec_t exec = nullptr;
-    for (const auto& g : b->graphs)
-      if (g.sig == sig) exec = g.exec;
-    if (!exec) {
+    auto pair_graph = [&](int active, hipGraphExec_t* out) -> int {
+      std::vector<long long> sig = {reinterpret_cast<long long>(b->dev), static_cast<long long>(o_tab),
+                                    static_cast<long long>(o_nbb), K, active, rk, gsym ? 1 : 0, fused ? 1 : 0,
+                                    static_cast<long long>(sizeof(T))};
+      for (int k = 0; k < BK_CLASSES; ++k) {
+        sig.push_back(static_cast<long long>(o_first[k]));
+        sig.push_back(static_cast<long long>(o_blk[k]));
+        sig.push_back(first[k][active]);
+      }
+      for (const auto& g : b->graphs)
+        if (g.sig == sig) {
+          *out = g.exec;
+          return SW_OK;
+        }
       if (!b->capture) SW_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
       SW_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
-      rc = substep(1, K, b->capture);
-      if (!rc) rc = substep(2, K, b->capture);
+      int rc2 = substep(1, active, b->capture);
+      if (!rc2) rc2 = substep(2, active, b->capture);
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(b->capture, &graph);
-      if (rc) {
+      if (rc2) {
         if (graph) (void)hipGraphDestroy(graph);
-        return rc;
+        return rc2;
       }
       if (e != hipSuccess) return fail(SW_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      hipGraphExec_t exec = nullptr;
       const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ei != hipSuccess) return fail(SW_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-      if (b->graphs.size() >= 8) {
-        (void)hipGraphExecDestroy(b->graphs.front().exec);
+      if (b->graphs.size() >= kBatchGraphs) {
+        // replays of the oldest graph may still be queued: destroy it once
+        // the work launched so far on `stream` has completed
+        hipEvent_t ev = nullptr;
+        SW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SW_HIP(hipEventRecord(ev, stream));
+        b->retired.push_back({b->graphs.front().exec, ev});
         b->graphs.erase(b->graphs.begin());
       }
       b->graphs.push_back({sig, exec});
+      *out = exec;
+      return SW_OK;
+    };
+    int k = 1, active = K;
+    while (k < nmax) {
+      while (active > 0 && nbb[active - 1] <= k) --active;
+      const int kend = nbb[active - 1];  // every member of the prefix steps through kend - 1
+      if ((k & 1) == 0 || kend - k < 2) {
+        if ((rc = substep(k, active, stream))) return rc;
+        ++k;
+        continue;
+      }
+      const int pairs = (kend - k) / 2;
+      hipGraphExec_t exec = nullptr;
+      if ((rc = pair_graph(active, &exec))) return rc;
+      for (int p = 0; p < pairs; ++p) SW_HIP(hipGraphLaunch(exec, stream));
+      k += 2 * pairs;
     }
-    for (int p = 0; p < (nmax - 1) / 2; ++p) SW_HIP(hipGraphLaunch(exec, stream));
-    if ((nmax - 1) % 2 && (rc = substep(nmax - 1, K, stream))) return rc;
   } else {
     int active = K;
     for (int k = 0; k < nmax; ++k) {
@@ -3401,14 +3452,17 @@ int sw_batch_create(sw_batch** out) {
 void sw_batch_destroy(sw_batch* b) {
   if (!b) return;
   if (b->used && b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->staged) {
-    (void)hipEventSynchronize(b->staged);
-    (void)hipEventDestroy(b->staged);
-  }
+  for (int k = 0; k < 2; ++k)
+    if (b->staged[k]) {
+      (void)hipEventSynchronize(b->staged[k]);
+      (void)hipEventDestroy(b->staged[k]);
+    }
+  b->reap(true);
   for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
   if (b->capture) (void)hipStreamDestroy(b->capture);
   if (b->dev) (void)hipFree(b->dev);
-  if (b->host) (void)hipHostFree(b->host);
+  for (int k = 0; k < 2; ++k)
+    if (b->host[k]) (void)hipHostFree(b->host[k]);
   delete b;
 }
 

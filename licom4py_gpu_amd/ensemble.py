@@ -62,8 +62,8 @@ class Batch:
         """Advance every member by its own nbb substeps; fills and returns each
         member's outputs (h0, ub, vb, then the 10 diagnostics unless
         ``diagnostics`` is false, when 3 output tensors suffice).  With
-        ``graph`` and one substep count for all members, the substeps after
-        the first replay a recorded hipGraph of substep pairs."""
+        ``graph`` the substeps after the first replay recorded hipGraphs of
+        substep pairs, one per number of members still stepping."""
         if not members:
             return []
         arr = (_hip.SWMember * len(members))()

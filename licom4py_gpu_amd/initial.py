@@ -143,10 +143,12 @@ def courant_dt(g: Dict[str, np.ndarray], h: np.ndarray, speed: np.ndarray,
     return float(courant * np.min(np.minimum(dx, dy) / c))
 
 
-def w92_case2(g: Dict[str, np.ndarray]):
-    """Reference W92 case 2 (initial/w92_field.py:53-91): (h0, ub, vb)."""
+def w92_case2(g: Dict[str, np.ndarray], ubar: float = 1.0):
+    """Reference W92 case 2 (initial/w92_field.py:53-91): (h0, ub, vb).  The
+    reference uses ubar = 1 m/s; Williamson et al. (1992) use
+    ubar = 2 pi a / 12 days (about 38.6 m/s)."""
     lon, lat = g["a_pt"][..., 0], g["a_pt"][..., 1]
-    ubar, alpha, radius, omega, grav = 1.0, 0.0, 6.371e6, 7.292e-5, 9.80
+    alpha, radius, omega, grav = 0.0, 6.371e6, 7.292e-5, 9.80
     u_lon = ubar * (np.cos(alpha) * np.cos(lat) + np.sin(alpha) * np.cos(lon) * np.sin(lat))
     u_lat = -ubar * np.sin(alpha) * np.sin(lon)
     l2c = g["a_l2c"]

@@ -132,3 +132,52 @@ def test_batch_separate_kernels_match_alone(dtype):
     for m in members:
         m.ctx.set_tuning(30, 30, 32)  # alone: the fused kernel
     _check_alone(torch, members)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_mixed_substeps_graph_replay_matches_alone(dtype):
+    """Different substep counts under SW_GRAPH: the substeps are replayed
+    from graphs recorded per number of members still stepping (with eager
+    substeps to reach odd starts), and every member still equals its run
+    alone; a second run of the same layout reuses the graphs."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    b = ensemble.Batch()
+    spec = [(8, 11, 40.0), (13, 8, 25.0), (16, 8, 31.0), (16, 5, 30.0), (10, 2, 33.0), (12, 1, 20.0)]
+    for seed in (21, 22):
+        members = _members(torch, tdt, spec, seed=seed)
+        b.run(members, graph=True)
+        _check_alone(torch, members)
+
+
+def test_batch_many_graph_layouts():
+    """More recorded layouts than the batch object keeps: evicted graphs are
+    destroyed only after the work queued before their eviction, and every run
+    stays equal to the members alone."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    b = ensemble.Batch()
+    for count in range(1, 21):
+        spec = [(8 + (k % 3), 3 + (k % 4), 30.0 + k) for k in range(count)]
+        members = _members(torch, torch.float64, spec, seed=100 + count)
+        b.run(members, diagnostics=False, graph=True)
+        if count % 5 == 0:
+            _check_alone(torch, members, diagnostics=False)
+    torch.cuda.synchronize()
+
+
+def test_nbb_zero_needs_skip_diag():
+    """nbb = 0 computes no diagnostics: with diagnostics requested the C-ABI
+    refuses (the reference's empty loop returns the input's diagnostics, which
+    the six input fields do not carry); without them h, u, v are copied."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    members = _members(torch, torch.float64, [(8, 0, 10.0), (9, 3, 10.0)])
+    with pytest.raises(RuntimeError, match="SW_SKIP_DIAG"):
+        ensemble.Batch().run(members)
+    with pytest.raises(RuntimeError, match="SW_SKIP_DIAG"):
+        members[0].ctx.rk(members[0].core6, 0, 10.0)
+    out = members[0].ctx.rk(members[0].core6, 0, 10.0, diagnostics=False)
+    for got, want in zip(out, (members[0].core6[0], members[0].core6[2], members[0].core6[3])):
+        assert torch.equal(got, want)
