@@ -108,14 +108,15 @@ def tune_batch(batch, members, candidates: Iterable[int] = CANDIDATES, substeps:
     every member of a batched run by timing the whole batch (short bands help
     a resolution alone, but in a batch the launches are full anyway and short
     bands only add warm-up rows).  Applies and returns it; the members' own
-    substep counts are restored."""
+    substep counts and output tensors are restored."""
     import torch
 
-    saved = [m.nbb for m in members]
+    saved = [(m.nbb, m.out) for m in members]
     times = {}
     try:
         for m in members:
             m.nbb = substeps
+            m.out = None  # the timed runs write their own scratch outputs
         for r in candidates:
             for fused in (False, True):
                 for m in members:
@@ -130,8 +131,8 @@ def tune_batch(batch, members, candidates: Iterable[int] = CANDIDATES, substeps:
                     best = min(best, time.perf_counter() - t0)
                 times[(r, fused)] = best
     finally:
-        for m, n in zip(members, saved):
-            m.nbb = n
+        for m, (n, out) in zip(members, saved):
+            m.nbb, m.out = n, out
     r, fused = min(times, key=times.get)
     for m in members:
         m.ctx.set_tuning(r, r, r if fused else 0)

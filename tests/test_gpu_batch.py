@@ -50,7 +50,9 @@ def _check_alone(torch, members, rk=2, diagnostics=True):
             assert np.array_equal(a, b), f"member {k} (C{m.ctx.nx}, nbb {m.nbb}) output {i} differs"
 
 
-SPEC = [(8, 5, 40.0), (13, 3, 25.0), (16, 0, 30.0), (16, 4, 31.0), (24, 1, 12.5), (10, 4, 33.0)]
+SPEC = [(8, 5, 40.0), (13, 3, 25.0), (16, 2, 30.0), (16, 4, 31.0), (24, 1, 12.5), (10, 4, 33.0)]
+# with a member that takes no substep (diagnostics off: nbb = 0 computes none)
+SPEC0 = SPEC[:2] + [(16, 0, 30.0)] + SPEC[3:]
 
 
 @pytest.mark.parametrize("rk", [2, 3])
@@ -97,7 +99,7 @@ def test_batch_object_reused_with_other_members():
     torch = _torch()
     from licom4py_gpu_amd import ensemble
     b = ensemble.Batch()
-    first = _members(torch, torch.float64, SPEC, seed=3)
+    first = _members(torch, torch.float64, SPEC0, seed=3)
     b.run(first, diagnostics=False)
     second = _members(torch, torch.float64, [(12, 6, 20.0), (9, 2, 44.0)], seed=5)
     b.run(second, diagnostics=False)
