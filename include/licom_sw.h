@@ -135,6 +135,18 @@ int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nb
  * interior max / min / mean, mymodule/diag.py:45-75). */
 int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, double out[2]);
 
+/* sw_integrals without allocating: the same kernels and the same result bits,
+ * with the per-block partial sums in a library-owned static device array
+ * (1024 blocks x 4 doubles = 32 KiB per GPU, held until the process ends,
+ * whatever the grid size: the kernel strides over the cells) and the two
+ * results written to `out`, device memory of 2 doubles, asynchronously on the
+ * context's stream (no host synchronisation).  Calls no hipMalloc / hipFree,
+ * so it may run under hipGraph stream capture or in a loop that must not
+ * allocate.  Calls that overlap on one GPU share the array and race.  No
+ * counterpart in the reference; for callers that cannot allocate (graph
+ * capture, serving loops). */
+int sw_integrals_noalloc(sw_ctx* ctx, const void* h, const void* u, const void* v, double* out);
+
 /* Rows per wavefront band of the stage kernels: rows_a / rows_b for the
  * separate height and velocity kernels (default 32), rows_f for the fused
  * kernel, which runs when rows_f > 0; rows_f = 0 selects the separate

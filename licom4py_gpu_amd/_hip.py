@@ -25,7 +25,7 @@ EXPORTED = (
     "sw_create", "sw_destroy", "sw_set_stream", "sw_rk2", "sw_rk3", "sw_stage",
     "sw_halo_scalar", "sw_halo_vector", "sw_edge_avg", "sw_sync",
     "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_get_tuning", "sw_layout_flags", "sw_profile", "sw_integrals",
-    "sw_create_like", "sw_batch_create", "sw_batch_destroy", "sw_batch_run",
+    "sw_create_like", "sw_batch_create", "sw_batch_destroy", "sw_batch_run", "sw_integrals_noalloc",
 )
 
 
@@ -80,6 +80,8 @@ def lib() -> ctypes.CDLL:
     L.sw_profile.restype = ci
     L.sw_integrals.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     L.sw_integrals.restype = ci
+    L.sw_integrals_noalloc.argtypes = [vp, vp, vp, vp, vp]
+    L.sw_integrals_noalloc.restype = ci
     L.sw_set_tuning.argtypes = [vp, ci, ci, ci]
     L.sw_set_tuning.restype = ci
     L.sw_get_tuning.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]

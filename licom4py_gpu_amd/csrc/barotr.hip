@@ -47,6 +47,7 @@
 #include <type_traits>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -2208,6 +2209,8 @@ struct GridMem {
 
 struct sw_ctx {
   int nx = 0, N = 0, dtype = SW_F64;
+  int device = 0;                // device current at sw_create
+  double* noalloc = nullptr;     // this device's address of the file-scope `scratch` array
   hipStream_t stream = nullptr;
   size_t cells = 0;        // 6 * N * N
   std::shared_ptr<GridMem> grid;  // owner of planes, locs and plan below
@@ -2845,6 +2848,55 @@ int run_integrals(sw_ctx* c, const void* h, const void* u, const void* v, double
   return SW_OK;
 }
 
+// Workspace of sw_integrals_noalloc: the per-block compensated partials of
+// k_integrals (4 doubles per block, kIntegralBlocks blocks).  Each GPU holds
+// its own copy until the process ends.  In host code the name `scratch` is
+// only a stand-in for the symbol: kernels get the device address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+// Calls on one GPU share the array, so calls that overlap on one GPU (two
+// streams, two threads) race; every slot a call reads is written by that
+// call's first kernel, so nothing needs clearing between calls.
+constexpr size_t kScratchDoubles = 4 * static_cast<size_t>(kIntegralBlocks);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_addr;  // per device ordinal
+
+// Device address of `scratch` on device `dev`, which must be current; looked
+// up once per device and kept.
+int scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0) return fail(SW_E_ARG, "bad device ordinal");
+  if (static_cast<size_t>(dev) >= g_scratch_addr.size()) g_scratch_addr.resize(dev + 1, nullptr);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SW_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return SW_OK;
+}
+
+// The integrals with the block partials in `scratch` and the result written
+// to device memory: no allocation, no host synchronisation.
+template <typename T>
+int run_integrals_noalloc(sw_ctx* c, const void* h, const void* u, const void* v, double* out) {
+  if (!c->noalloc) return fail(SW_E_STATE, "context has no scratch address (sw_create did not resolve it)");
+  const size_t need = 4 * static_cast<size_t>(kIntegralBlocks);
+  if (need > kScratchDoubles)
+    return fail(SW_E_ARG, "block partials (" + std::to_string(need) + " doubles) exceed the scratch array (" +
+                              std::to_string(kScratchDoubles) + ")");
+  IntegralArgs<T> a{geo_of(c), planes_of<T>(c), static_cast<const T*>(h), static_cast<const T*>(u),
+                    static_cast<const T*>(v), c->noalloc};
+  hipLaunchKernelGGL(k_integrals<T>, dim3(kIntegralBlocks), dim3(256), 0, c->stream, a);
+  SW_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_integrals_final, dim3(1), dim3(256), 0, c->stream, c->noalloc, kIntegralBlocks, out);
+  SW_HIP(hipGetLastError());
+  return SW_OK;
+}
+
 // ---------------------------------------------------------------------------
 // batched runs
 // ---------------------------------------------------------------------------
@@ -3229,8 +3281,11 @@ int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CON
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 0;
     c->rows_f = default_rows_f(nx, cus, dtype);
+    c->device = dev;
   }
-  rc = validate_consts(c, consts);
+  // this device's address of the no-allocation workspace (once per device)
+  rc = scratch_address(c->device, &c->noalloc);
+  if (!rc) rc = validate_consts(c, consts);
   if (!rc) rc = (dtype == SW_F64) ? do_create_planes<double>(c, consts) : do_create_planes<float>(c, consts);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(c->stream);
@@ -3259,6 +3314,8 @@ int sw_create_like(sw_ctx** out, const sw_ctx* base) {
   c->dtype = base->dtype;
   c->stream = base->stream;
   c->cells = base->cells;
+  c->device = base->device;
+  c->noalloc = base->noalloc;
   c->grid = base->grid;
   c->planes = base->planes;
   c->locs = base->locs;
@@ -3409,6 +3466,12 @@ int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, doubl
   if (!ctx || !h || !u || !v || !out) return fail(SW_E_ARG, "null argument");
   return ctx->dtype == SW_F64 ? run_integrals<double>(ctx, h, u, v, out)
                               : run_integrals<float>(ctx, h, u, v, out);
+}
+
+int sw_integrals_noalloc(sw_ctx* ctx, const void* h, const void* u, const void* v, double* out) {
+  if (!ctx || !h || !u || !v || !out) return fail(SW_E_ARG, "null argument");
+  return ctx->dtype == SW_F64 ? run_integrals_noalloc<double>(ctx, h, u, v, out)
+                              : run_integrals_noalloc<float>(ctx, h, u, v, out);
 }
 
 int sw_get_tuning(const sw_ctx* ctx, int* rows_a, int* rows_b, int* rows_f) {
