@@ -8,14 +8,21 @@ in HBM before the timed region.  The reported value is interior cells x
 substeps / wall time (cell-substeps/s).
 
 The JSON line also carries
-  roofline       the dominant kernel's algorithmic HBM bytes per launch over
-                 its average launch time, measured here with start / stop
-                 hipEvents attached to each dispatch on the launch stream
-                 (hipExtLaunchKernelGGL) in an eager pass of the same
-                 workload, kernels in their step order, and its HBM traffic
-                 per launch from the committed rocprofv3 counter summary,
+  roofline       the dominant kernel (the fused stage kernel) against the
+                 path's algorithmic bytes per stage launch, 148 B per interior
+                 cell in FP64 (296 B per cell-substep over the two stages of
+                 a substep), over its average launch time as it runs in the
+                 timed steps: the substep-pair graph the timed steps replay,
+                 recorded with an event node around every kernel node and
+                 replayed after the timed region; plus the kernel's own byte
+                 count (face-shared planes counted once for six faces) and
+                 its L2-to-fabric bytes per launch from the committed
+                 rocprofv3 counter summary,
   roofline_path  the whole step against 296 B per FP64 cell-substep (148 B for
                  FP32) at 8 TB/s,
+  fp32           the same workload in the FP32 form: timed steps, its path
+                 roofline fraction, its speed-up over FP64, and its height
+                 error against FP64 after a fixed number of substeps,
   cpu_baseline   the NumPy FP64 oracle on one host core over a bounded sample
                  of the same workload,
   serving        p50/p99 latency and throughput of served solves (a seeded
@@ -65,8 +72,12 @@ def parse():
     p.add_argument("--no-graph", action="store_true", help="launch every substep eagerly")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default band height instead of timing the candidates")
-    p.add_argument("--profile-substeps", type=int, default=10,
-                   help="substeps of the event-timed eager pass (0 disables)")
+    p.add_argument("--profile-replays", type=int, default=4,
+                   help="replays of the event-timed substep-pair graph (0 disables)")
+    p.add_argument("--fp32-steps", type=int, default=None,
+                   help="timed steps of the FP32 leg (default --steps; 0 disables)")
+    p.add_argument("--accuracy-substeps", type=int, default=10000,
+                   help="substeps after which the FP32 height is compared with FP64")
     p.add_argument("--cpu-substeps", type=int, default=8,
                    help="substeps of the CPU oracle sample (0 disables)")
     p.add_argument("--cpu-nx", type=int, default=None, help="resolution of the CPU sample (default --nx)")
@@ -142,8 +153,7 @@ def main() -> int:
         tuning = {k: tuning[k] for k in ("rows_a", "rows_b", "rows_f", "table")}
     setup_s = time.perf_counter() - t_setup
 
-    # served solves first: the timed steps and the event-timed pass are then
-    # the last stage-kernel launches of the run (a kernel trace splits them off)
+    # served solves first (they run both precisions at many resolutions)
     serving = None
     if args.serve_requests > 0:
         from licom4py_gpu_amd import serving as srv
@@ -160,6 +170,15 @@ def main() -> int:
         if not rep["finite"]:
             print("bench.py: non-finite served result", file=sys.stderr)
             return 1
+
+    # the FP32 form next (its accuracy reference runs FP64 too), so the FP64
+    # timed steps and their event-timed graph pass are the last FP64
+    # stage-kernel launches of the run
+    fp32 = None
+    fp32_steps = args.steps if args.fp32_steps is None else args.fp32_steps
+    if args.dtype == "f64" and fp32_steps > 0 and not args.no_graph:
+        fp32 = fp32_leg(consts, state, dt, nbb, fp32_steps, args.warmup, args.accuracy_substeps,
+                        args.profile_replays, not args.no_autotune, 6 * nx * nx, f"C{nx}_f32")
 
     def step(s):
         return barotr.barotr_rk2_core(s, consts, nbb, dt) if not args.no_graph else \
@@ -185,31 +204,15 @@ def main() -> int:
     path_bytes = PATH_BYTES[args.dtype]
     workload = f"C{nx}_{args.dtype}"
 
-    roofline = None
-    kernels = None
-    if args.profile_substeps > 0:
-        prof = ctx.profile(state[:6], args.profile_substeps, dt)
-        kernels = {k: {"ms_total": round(ms, 4), "launches": n,
-                       "us_avg": round(ms / n * 1e3, 3) if n else None}
-                   for k, (ms, n) in prof.items()}
-        dom = max(("march_a", "march_b", "fused"), key=lambda k: prof[k][0])
-        kernels["timing"] = "start/stop hipEvents on each dispatch (hipExtLaunchKernelGGL), eager pass"
-        ms, n = prof[dom]
-        avg_s = ms / n * 1e-3
-        esize = 8 if args.dtype == "f64" else 4
-        alg = VALUES_PER_CELL[dom] * esize * cells
-        achieved = alg / avg_s / 1e9
-        traffic = load_traffic(dom, workload)
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                    "traffic": traffic, "algorithmic_bytes_per_launch": int(alg),
-                    "avg_launch_us": round(avg_s * 1e6, 3),
-                    # measured HBM bytes (counters) over the same launch time
-                    "traffic_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None}
+    roofline, kernels = graph_roofline(ctx, state, dt, args.profile_replays, args.dtype, cells, workload) \
+        if args.profile_replays > 0 else (None, None)
     path_ach = value * path_bytes / 1e9
     roofline_path = {"bound": "hbm", "achieved": round(path_ach, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(path_ach / PEAK_HBM_GBS, 4),
                      "bytes_per_cell_substep": path_bytes}
+
+    if fp32 is not None:
+        fp32["ratio_to_fp64"] = fp32["value"] / value
 
     cpu = None
     if args.cpu_substeps > 0:
@@ -236,11 +239,95 @@ def main() -> int:
         "roofline": roofline,
         "roofline_path": roofline_path,
         "kernels": kernels,
+        "fp32": fp32,
         "cpu_baseline": cpu,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+KERNEL_TIMING = ("device wall clock at every wave's entry and exit (first entry to last exit per launch) in the "
+                 "recorded substep-pair graph the timed steps replay (16 pairs), summed over replays after the "
+                 "timed region")
+
+
+def graph_roofline(ctx, state, dt, replays, dtype, cells, workload):
+    """The fused stage kernel against the path's bytes per stage launch, timed
+    as it runs in the timed steps' graph replays."""
+    if ctx.tuning[2] <= 0:
+        return None, {"timing": "the tuned path runs the separate stage kernels; the graph timer covers the fused one"}
+    prof = ctx.profile_graph(state[:6], 16, replays, dt)
+    (fms, n), (gms, nrep) = prof["fused"], prof["graph"]
+    substeps = 32 * nrep
+    kernels = {"fused": {"ms_total": round(fms, 4), "launches": n, "us_avg": round(fms / n * 1e3, 3)},
+               "graph": {"ms_total": round(gms, 4), "replays": nrep, "us_per_substep": round(gms / substeps * 1e3, 3),
+                         "fused_share": round(fms / gms, 4)},
+               "timing": KERNEL_TIMING}
+    esize = 8 if dtype == "f64" else 4
+    avg_s = fms / n * 1e-3
+    per_cell = PATH_BYTES[dtype] / 2.0  # the path's bytes per cell and stage launch
+    alg = per_cell * cells
+    achieved = alg / avg_s / 1e9
+    traffic = load_traffic("fused", workload)
+    own = VALUES_PER_CELL["fused"] * esize * cells
+    roofline = {"bound": "hbm", "kernel": "fused", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "traffic_kind": "L2-to-fabric bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a calibration "
+                                "copy); the 256 MB MALL serves part of them, so they bound HBM bytes from above",
+                "algorithmic_bytes_per_launch": int(alg), "bytes_per_cell_per_launch": per_cell,
+                "avg_launch_us": round(avg_s * 1e6, 3), "timing": KERNEL_TIMING,
+                # the kernel's own count: face-shared planes once for the six faces, a_gct as two planes
+                "kernel_bytes_per_launch": int(own), "kernel_frac": round(own / avg_s / 1e9 / PEAK_HBM_GBS, 4),
+                "traffic_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None}
+    return roofline, kernels
+
+
+def fp32_leg(consts, state, dt, nbb, steps, warmup, acc_substeps, replays, autotune_on, cells, workload):
+    """The same workload in the FP32 form: timed steps, the graph-timed
+    roofline of its stage kernel, and its height against FP64."""
+    import torch
+    from licom4py_gpu_amd import barotr
+
+    st32 = tuple(x.to(torch.float32) for x in state)
+    ctx32 = barotr.context_for(consts, torch.float32)
+    if autotune_on:
+        from licom4py_gpu_amd import autotune
+        autotune.tune(ctx32, st32[:6], dt, use_cache=False)
+    s = st32
+    for _ in range(warmup):
+        s = barotr.barotr_rk2_core(s, consts, nbb, dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = barotr.barotr_rk2_core(s, consts, nbb, dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    v32 = cells * nbb * steps / el
+    ach = v32 * PATH_BYTES["f32"] / 1e9
+    out = {"value": v32, "unit": "cell-substeps/s", "steps": steps, "warmup": warmup, "ms_per_step": el / steps * 1e3,
+           "ratio_to_fp64": None,  # filled in once the FP64 steps are timed
+           "roofline_path": {"achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(ach / PEAK_HBM_GBS, 4), "bytes_per_cell_substep": PATH_BYTES["f32"]},
+           "tuning": dict(zip(("rows_a", "rows_b", "rows_f"), ctx32.tuning))}
+    if replays > 0:
+        out["roofline"], _ = graph_roofline(ctx32, st32, dt, replays, "f32", cells, workload)
+    # accuracy: FP32 against FP64 from the same initial state (binary64
+    # compensated mass integrals for the drift)
+    ctx64 = barotr.context_for(consts, torch.float64)
+    h64 = barotr.barotr_rk2_core(state, consts, acc_substeps, dt)
+    h32 = barotr.barotr_rk2_core(st32, consts, acc_substeps, dt)
+    inner = (slice(None), slice(3, -3), slice(3, -3))
+    rel = [float(torch.linalg.vector_norm(h32[k][inner].double() - h64[k][inner]) /
+                 torch.linalg.vector_norm(h64[k][inner])) for k in (0, 2, 3)]
+    m0 = ctx32.integrals(st32[0], st32[2], st32[3])[0]
+    m1 = ctx32.integrals(h32[0], h32[2], h32[3])[0]
+    m64 = ctx64.integrals(h64[0], h64[2], h64[3])[0]
+    m64_0 = ctx64.integrals(state[0], state[2], state[3])[0]
+    out.update({"substeps_compared": acc_substeps, "height_rel_l2_vs_fp64": rel[0],
+                "u_rel_l2_vs_fp64": rel[1], "v_rel_l2_vs_fp64": rel[2],
+                "mass_drift_rel_fp32": (m1 - m0) / m0, "mass_drift_rel_fp64": (m64 - m64_0) / m64_0})
+    return out
 
 
 def _eager(ctx, s, nbb, dt):

@@ -126,6 +126,22 @@ int sw_routing_table(int32_t out[6 * 4 * 3]);
 int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nbb,
                double dtb, int reps, double ms[SW_N_KERNEL_CLASSES], int counts[SW_N_KERNEL_CLASSES]);
 
+/* The fused stage kernel as it runs in the recorded substep-pair graphs of
+ * sw_rk2's SW_GRAPH path (which event nodes inside a graph cannot time): the
+ * first RK2 substep runs eagerly (untimed) from in[] into the context's
+ * buffer, then a graph of `pairs` substep pairs is recorded with the fused
+ * kernel's launch stamps on (lane 0 of every wave writes the device wall
+ * clock at entry and exit; a launch lasts from its first entry to its last
+ * exit) and replayed `replays` times, the host reading the stamps after each
+ * replay.  ms[6] / counts[6] (the fused class of sw_profile) sum the launch
+ * durations over every replay, the other classes stay 0; *graph_ms sums the
+ * replays (events around each graph launch).  out[0..2] receive h, u, v after
+ * 1 + 2 * pairs * replays substeps.  Needs the fused form (rows_f > 0).  A
+ * measurement helper with no counterpart in the reference. */
+int sw_profile_graph(sw_ctx* ctx, const void* const in[6], void* const out[13], int pairs, int replays,
+                     double dtb, double ms[SW_N_KERNEL_CLASSES], int counts[SW_N_KERNEL_CLASSES],
+                     double* graph_ms);
+
 /* Global integrals of a state in binary64 with compensated summation over
  * the interior cells, whatever the state precision:
  *   out[0] mass   = sum (dzph + h) * da

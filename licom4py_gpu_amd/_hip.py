@@ -26,6 +26,7 @@ EXPORTED = (
     "sw_halo_scalar", "sw_halo_vector", "sw_edge_avg", "sw_sync",
     "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_get_tuning", "sw_layout_flags", "sw_profile", "sw_integrals",
     "sw_create_like", "sw_batch_create", "sw_batch_destroy", "sw_batch_run", "sw_integrals_noalloc",
+    "sw_profile_graph",
 )
 
 
@@ -78,6 +79,10 @@ def lib() -> ctypes.CDLL:
     L.sw_profile.argtypes = [vp, vpp, vpp, ci, ctypes.c_double, ci,
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ci)]
     L.sw_profile.restype = ci
+    L.sw_profile_graph.argtypes = [vp, vpp, vpp, ci, ci, ctypes.c_double,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ci),
+                                   ctypes.POINTER(ctypes.c_double)]
+    L.sw_profile_graph.restype = ci
     L.sw_integrals.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     L.sw_integrals.restype = ci
     L.sw_integrals_noalloc.argtypes = [vp, vp, vp, vp, vp]

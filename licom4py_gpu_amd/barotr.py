@@ -178,6 +178,25 @@ class SWContext:
         self.last_profile_out = outs  # the pass computes the same outputs as rk()
         return {name: (ms[k], cnt[k]) for k, name in enumerate(_hip.KERNEL_CLASSES)}
 
+    def profile_graph(self, core6: Sequence[torch.Tensor], pairs: int, replays: int, dtb: float):
+        """The fused kernel as it runs in the recorded substep-pair graphs: a
+        graph of `pairs` RK2 substep pairs with the kernel's launch stamps on,
+        replayed `replays` times (after one eager substep); returns
+        {"fused": (total ms, launches), "graph": (total ms, replays)}."""
+        ins = [self._check_field(x, STATE_KEYS[k]) for k, x in enumerate(core6)]
+        outs = self.empty(3)
+        self._bind_stream()
+        pin, _k1 = _hip.ptr_array([x.data_ptr() for x in ins])
+        pout, _k2 = _hip.ptr_array([x.data_ptr() for x in outs] + [0] * 10)
+        ms = (ctypes.c_double * len(_hip.KERNEL_CLASSES))()
+        cnt = (ctypes.c_int * len(_hip.KERNEL_CLASSES))()
+        gms = ctypes.c_double()
+        _hip.check(_hip.lib().sw_profile_graph(self._h, pin, pout, int(pairs), int(replays), float(dtb), ms, cnt,
+                                               ctypes.byref(gms)), "sw_profile_graph")
+        self.last_profile_out = outs
+        k = _hip.KERNEL_CLASSES.index("fused")
+        return {"fused": (ms[k], cnt[k]), "graph": (gms.value, int(replays))}
+
     def integrals(self, h: torch.Tensor, u: torch.Tensor, v: torch.Tensor):
         """(mass, energy) in binary64 with compensated summation (see
         include/licom_sw.h); waits for the result."""

@@ -418,6 +418,9 @@ struct MarchArgs {
   int shared;  // rdx, rdy, rda, c_dy, d_dx identical on every face
   int gsym;    // a_gct symmetric with equal diagonal
   T* sink;     // 256 elements absorbing the stores of inactive lanes
+  // null, or 2 wall-clock stamps (entry, exit) per wave of the launch, written
+  // by the fused kernel's lane 0 (launch-duration measurement in graphs)
+  unsigned long long* stamps;
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -1343,15 +1346,29 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   if (n == cnt) step(n, pre, alt);
 }
 
+// Launch stamps (sw_profile_graph): lane 0 of each wave writes the device
+// wall clock to slot 2 w (entry) or 2 w + 1 (exit) of a.stamps, w the
+// hardware wave index of the launch; nothing is held across the march.
+template <typename T>
+__device__ __forceinline__ void stamp(const MarchArgs<T>& a, int which) {
+  if (a.stamps && (threadIdx.x & 63) == 0)
+    a.stamps[2 * (static_cast<size_t>(blockIdx.x) * 4 + (threadIdx.x >> 6)) + which] = wall_clock64();
+}
+
 template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void fused_body(const MarchArgs<T>& a, int L, FRing<T>* rings) {
+  stamp(a, 0);
   WaveTile w;
-  if (!decode_fwave(a, L, w)) return;
+  if (!decode_fwave(a, L, w)) {
+    stamp(a, 1);
+    return;
+  }
   FRing<T>& rg = rings[w.wslot];
   if ((w.r0 - HALO) / a.R & 1)
     fused_march<T, GSYM, SAMEP, true>(a, w, rg);
   else
     fused_march<T, GSYM, SAMEP, false>(a, w, rg);
+  stamp(a, 1);
 }
 
 template <typename T, bool GSYM, bool SAMEP>
@@ -2241,6 +2258,10 @@ struct sw_ctx {
   };
   std::vector<ProfRec>* prof = nullptr;  // set only by sw_profile
   int prof_reps = 1;                     // back-to-back launches per timed launch
+  // launch-duration stamps of the fused kernel (sw_profile_graph only): the
+  // buffer, its next free slot and end, and (first slot, waves) per launch
+  mutable unsigned long long *stamp_base = nullptr, *stamp_next = nullptr, *stamp_end = nullptr;
+  mutable std::vector<std::pair<size_t, int>> stamp_launches;
 };
 
 // Reusable state of sw_batch_run: the device copy of the member tables and
@@ -2490,6 +2511,13 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
     const bool sf = A.mf.hp == A.mf.h && A.mf.up == A.mf.u && A.mf.vp == A.mf.v;
     const int nft = 6 * frame_tiles(c->nx);
     const dim3 gf((A.mf.nwaves + 3) / 4), gr(nft + static_cast<int>(blocks_for(c->plan_n)));
+    if (c->stamp_next) {  // launch-duration stamps (sw_profile_graph)
+      const int waves = 4 * static_cast<int>(gf.x);
+      if (c->stamp_next + 2 * waves > c->stamp_end) return fail(SW_E_STATE, "stamp buffer too small");
+      A.mf.stamps = c->stamp_next;
+      c->stamp_launches.push_back({static_cast<size_t>(c->stamp_next - c->stamp_base), waves});
+      c->stamp_next += 2 * waves;
+    }
     auto lf = [&](auto k) { return launch(c, K_FUSED, k, gf, A.mf); };
     if (c->gsym)
       rc = sf ? lf(k_fused<T, true, true>) : lf(k_fused<T, true, false>);
@@ -2701,6 +2729,109 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
     }
   }
   if (!(flags & SW_SKIP_DIAG)) return run_diag<T>(c, dh, du, dv, out, rk - 1);
+  return SW_OK;
+}
+
+// The substep-pair graph of run_rk's SW_GRAPH path, recorded with the fused
+// kernel's launch-duration stamps on (each wave's lane 0 writes the device
+// wall clock at entry and exit), replayed `replays` times.  After each replay
+// the host reads the stamps: a launch lasts from its first wave's entry to
+// its last wave's exit.  ms[K_FUSED] / counts[K_FUSED] sum these over every
+// replay; *graph_ms sums the replays themselves (events around each graph
+// launch).  The first substep, from the caller's fields into X, runs eagerly
+// and untimed; out[0..2] receive the state after 1 + 2 pairs replays
+// substeps.
+template <typename T>
+int run_profile_graph(sw_ctx* c, const void* const in[6], void* const out[13], int pairs, int replays,
+                      double dtb, double* ms, int* counts, double* graph_ms) {
+  int rc = check_ptrs(in, 6, "in");
+  if (!rc) rc = check_ptrs(const_cast<const void* const*>(out), 3, "out");
+  if (!rc) rc = check_disjoint(c, in, out);
+  if (rc) return rc;
+  if (c->rows_f <= 0) return fail(SW_E_STATE, "graph timing covers the fused stage kernel (rows_f > 0)");
+  for (int k = 0; k < K_CLASSES; ++k) {
+    ms[k] = 0.0;
+    counts[k] = 0;
+  }
+  *graph_ms = 0.0;
+  int rate_khz = 0;
+  SW_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device));
+  if (rate_khz <= 0) return fail(SW_E_HIP, "device reports no wall-clock rate");
+  const T* p[6];
+  for (int k = 0; k < 6; ++k) p[k] = static_cast<const T*>(in[k]);
+  T* X[3] = {scratch_field<T>(c, 6), scratch_field<T>(c, 7), scratch_field<T>(c, 8)};
+  T* Y[3] = {scratch_field<T>(c, 9), scratch_field<T>(c, 10), scratch_field<T>(c, 11)};
+  if ((rc = run_substep<T>(c, 2, dtb, p[0], p[1], p[2], p[3], p[4], p[5], X, true))) return rc;
+  // stamp slots: 2 per wave of each of the graph's 4 pairs fused launches
+  const size_t waves = 4 * static_cast<size_t>((6 * ((c->nx + kFStride - 1) / kFStride) *
+                                                ((c->nx + c->rows_f - 1) / c->rows_f) + 3) / 4);
+  const size_t nslots = 2 * waves * 4 * static_cast<size_t>(pairs);
+  unsigned long long* dstamps = nullptr;
+  SW_HIP(hipMalloc(&dstamps, nslots * sizeof(unsigned long long)));
+  std::vector<unsigned long long> hstamps(nslots);
+  if (!c->capture) rc = hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking) == hipSuccess
+                            ? SW_OK : fail(SW_E_HIP, "hipStreamCreateWithFlags failed");
+  hipStream_t user = c->stream;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (!rc && hipStreamSynchronize(user) != hipSuccess) rc = fail(SW_E_HIP, "hipStreamSynchronize failed");
+  if (!rc) {
+    c->stream = c->capture;
+    c->stamp_base = c->stamp_next = dstamps;
+    c->stamp_end = dstamps + nslots;
+    c->stamp_launches.clear();
+    hipError_t e = hipStreamBeginCapture(c->capture, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) rc = fail(SW_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    for (int q = 0; q < pairs && !rc; ++q) {
+      rc = run_substep<T>(c, 2, dtb, X[0], X[0], X[1], X[2], X[1], X[2], Y);
+      if (!rc) rc = run_substep<T>(c, 2, dtb, Y[0], Y[0], Y[1], Y[2], Y[1], Y[2], X);
+    }
+    if (e == hipSuccess) {
+      e = hipStreamEndCapture(c->capture, &graph);
+      if (!rc && e != hipSuccess) rc = fail(SW_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    c->stream = user;
+    c->stamp_base = c->stamp_next = c->stamp_end = nullptr;
+  }
+  if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = fail(SW_E_HIP, "hipGraphInstantiate failed");
+  if (!rc && (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess))
+    rc = fail(SW_E_HIP, "hipEventCreate failed");
+  for (int r = 0; r < replays && !rc; ++r) {
+    hipError_t e = hipEventRecord(ea, c->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ea, eb);
+    if (e == hipSuccess)
+      e = hipMemcpy(hstamps.data(), dstamps, nslots * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(SW_E_HIP, std::string("timed graph replay: ") + hipGetErrorString(e));
+      break;
+    }
+    *graph_ms += t;
+    for (const auto& L : c->stamp_launches) {
+      unsigned long long lo = ~0ull, hi = 0;
+      for (int w = 0; w < L.second; ++w) {
+        lo = std::min(lo, hstamps[L.first + 2 * w]);
+        hi = std::max(hi, hstamps[L.first + 2 * w + 1]);
+      }
+      ms[K_FUSED] += static_cast<double>(hi - lo) / rate_khz;
+      counts[K_FUSED] += 1;
+    }
+  }
+  c->stamp_launches.clear();
+  if (ea) (void)hipEventDestroy(ea);
+  if (eb) (void)hipEventDestroy(eb);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipFree(dstamps);
+  if (rc) return rc;
+  for (int k = 0; k < 3; ++k)
+    SW_HIP(hipMemcpyAsync(out[k], X[k], c->cells * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+  SW_HIP(hipStreamSynchronize(c->stream));
   return SW_OK;
 }
 
@@ -3460,6 +3591,17 @@ int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nb
   if (rc) return rc;
   if (e != hipSuccess) return fail(SW_E_HIP, std::string("sw_profile: ") + hipGetErrorString(e));
   return SW_OK;
+}
+
+int sw_profile_graph(sw_ctx* ctx, const void* const in[6], void* const out[13], int pairs, int replays,
+                     double dtb, double ms[SW_N_KERNEL_CLASSES], int counts[SW_N_KERNEL_CLASSES],
+                     double* graph_ms) {
+  if (!ctx || !ms || !counts || !graph_ms) return fail(SW_E_ARG, "null argument");
+  if (pairs < 1 || pairs > 64) return fail(SW_E_ARG, "pairs must be in [1, 64]");
+  if (replays < 1 || replays > 10000) return fail(SW_E_ARG, "replays must be in [1, 10000]");
+  if (!(dtb == dtb) || dtb > 1e300 || dtb < -1e300) return fail(SW_E_ARG, "dtb must be finite");
+  return ctx->dtype == SW_F64 ? run_profile_graph<double>(ctx, in, out, pairs, replays, dtb, ms, counts, graph_ms)
+                              : run_profile_graph<float>(ctx, in, out, pairs, replays, dtb, ms, counts, graph_ms);
 }
 
 int sw_integrals(sw_ctx* ctx, const void* h, const void* u, const void* v, double out[2]) {
