@@ -150,7 +150,9 @@ def main() -> int:
     if not args.no_autotune:
         from licom4py_gpu_amd import autotune
         tuning = autotune.tune(ctx, state[:6], dt, use_cache=False)
-        tuning = {k: tuning[k] for k in ("rows_a", "rows_b", "rows_f", "table")}
+        tuning = {k: tuning[k] for k in ("rows_a", "rows_b", "rows_f", "rows_f2", "split_table_us", "table")}
+    else:
+        tuning = {"rows_a": ctx.tuning[0], "rows_b": ctx.tuning[1], "rows_f": ctx.tuning[2], "rows_f2": ctx.split}
     setup_s = time.perf_counter() - t_setup
 
     # served solves first (they run both precisions at many resolutions)
@@ -309,7 +311,7 @@ def fp32_leg(consts, state, dt, nbb, steps, warmup, acc_substeps, replays, autot
            "ratio_to_fp64": None,  # filled in once the FP64 steps are timed
            "roofline_path": {"achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": round(ach / PEAK_HBM_GBS, 4), "bytes_per_cell_substep": PATH_BYTES["f32"]},
-           "tuning": dict(zip(("rows_a", "rows_b", "rows_f"), ctx32.tuning))}
+           "tuning": dict(zip(("rows_a", "rows_b", "rows_f", "rows_f2"), (*ctx32.tuning, ctx32.split)))}
     if replays > 0:
         out["roofline"], _ = graph_roofline(ctx32, st32, dt, replays, "f32", cells, workload)
     # accuracy: FP32 against FP64 from the same initial state (binary64

@@ -132,11 +132,13 @@ int sw_profile(sw_ctx* ctx, const void* const in[6], void* const out[13], int nb
  * buffer, then a graph of `pairs` substep pairs is recorded with the fused
  * kernel's launch stamps on (lane 0 of every wave writes the device wall
  * clock at entry and exit; a launch lasts from its first entry to its last
- * exit) and replayed `replays` times, the host reading the stamps after each
- * replay.  ms[6] / counts[6] (the fused class of sw_profile) sum the launch
- * durations over every replay, the other classes stay 0; *graph_ms sums the
- * replays (events around each graph launch).  out[0..2] receive h, u, v after
- * 1 + 2 * pairs * replays substeps.  Needs the fused form (rows_f > 0).  A
+ * exit) and replayed: `replays` measured replays, each after 3 unmeasured
+ * ones queued back to back (so the device runs at the clocks of a sustained
+ * sequence of replays), the host reading the stamps after each measured one.
+ * ms[6] / counts[6] (the fused class of sw_profile) sum the launch durations
+ * over the measured replays, the other classes stay 0; *graph_ms sums the
+ * measured replays (events around each graph launch).  out[0..2] receive
+ * h, u, v after 1 + 2 * pairs * 4 * replays substeps.  Needs the fused form (rows_f > 0).  A
  * measurement helper with no counterpart in the reference. */
 int sw_profile_graph(sw_ctx* ctx, const void* const in[6], void* const out[13], int pairs, int replays,
                      double dtb, double ms[SW_N_KERNEL_CLASSES], int counts[SW_N_KERNEL_CLASSES],
@@ -176,6 +178,18 @@ int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b, int rows_f);
 
 /* The context's current band heights (sw_set_tuning).  Build-defined. */
 int sw_get_tuning(const sw_ctx* ctx, int* rows_a, int* rows_b, int* rows_f);
+
+/* Split bands of the fused kernel (sw_rk2 / sw_rk3 / sw_stage on one
+ * context; batched runs keep uniform bands).  A launch fills the device in
+ * one round of waves, and the older of two waves sharing a SIMD gets about
+ * twice the issue slots, so with uniform bands the second block on every CU
+ * finishes far later than the first.  With rows_short > 0 the columns of
+ * each face alternate bands of rows_f rows (marched by the first blocks
+ * dispatched, at most one per CU) and rows_short rows (the later blocks), so
+ * both finish together.  0 restores uniform bands.  Performance only:
+ * results are bit-identical for every setting.  Build-defined. */
+int sw_set_band_split(sw_ctx* ctx, int rows_short);
+int sw_get_band_split(const sw_ctx* ctx, int* rows_short);
 
 /* Host-only: what sw_create detected in the consts.  Bit 0: dzph_x/dzph_y
  * uniform (read as scalars); bit 1: rdx, rdy, rda, c_dy, d_dx identical on all

@@ -7,8 +7,11 @@ stencil warm-up rows every band repeats.  `tune` times a short eager run
 (per-dispatch hipEvents) for every candidate R of the separate height and
 velocity kernels and of the fused kernel (with its frame kernel) on the
 context's own resolution and precision, keeps the fastest band heights and
-the faster of the two paths, and records them in a JSON cache keyed by (nx,
-dtype, device name); `tune_batch` picks one R for all members of a batched
+the faster of the two paths; for the fused path it then times the fused
+kernel's heights again as it runs in graph replays (eager timing does not
+reproduce them) and tries split bands around the best one (the first blocks
+dispatched march taller bands).  It records the result in a JSON cache keyed
+by (nx, dtype, device name); `tune_batch` picks one R for all members of a batched
 run by timing the whole batch.  Results do not depend on R or on the path
 (every setting produces identical bits), so tuning is free to change them.
 """
@@ -22,7 +25,8 @@ from typing import Dict, Iterable, Optional, Sequence, Tuple
 
 CANDIDATES: Tuple[int, ...] = (8, 12, 16, 20, 24, 26, 28, 30, 32, 36, 40, 48)
 CACHE_ENV = "LICOM_SW_TUNE_CACHE"
-_KERNEL_SET = "fused2"  # cache entries of other kernel generations are ignored
+_KERNEL_SET = "fused3"  # cache entries of other kernel generations are ignored
+SPLIT_SHARES = {"float64": (0.62, 0.66, 0.69, 0.72), "float32": (0.55, 0.6, 0.65)}
 
 
 def _cache_path() -> str:
@@ -64,7 +68,9 @@ def tune(ctx, core6: Sequence, dt: float, substeps: int = 2, reps: int = 2,
     if key in cache:
         best = cache[key]
         ctx.set_tuning(best["rows_a"], best["rows_b"], best["rows_f"])
+        ctx.set_band_split(best.get("rows_f2", 0))
         return dict(best, cached=True)
+    ctx.set_band_split(0)  # uniform bands while the heights are swept
     cands = [c for c in candidates if c <= max(4, ctx.nx)]
     times = {}
 
@@ -92,14 +98,52 @@ def tune(ctx, core6: Sequence, dt: float, substeps: int = 2, reps: int = 2,
     rf = min(cands, key=lambda r: times[r][2])
     use_fused = times[rf][2] < times[ra][0] + times[rb][1]
     ctx.set_tuning(ra, rb, rf if use_fused else 0)
-    result = {"rows_a": ra, "rows_b": rb, "rows_f": rf if use_fused else 0,
-              "ms_a": times[ra][0], "ms_b": times[rb][1], "ms_fused_frame": times[rf][2],
+    rf_uniform, rf2, split_table = rf, 0, {}
+    if use_fused:
+        rf, rf2, split_table = _tune_fused_graph(ctx, core6, dt, ra, rb, cands)
+    result = {"rows_a": ra, "rows_b": rb, "rows_f": rf if use_fused else 0, "rows_f2": rf2,
+              "ms_a": times[ra][0], "ms_b": times[rb][1], "ms_fused_frame": times[rf_uniform][2],
               "table": {str(r): {"ms_a": t[0], "ms_b": t[1], "ms_fused_frame": t[2]} for r, t in times.items()},
+              "split_table_us": split_table,
               "when": time.strftime("%Y-%m-%d %H:%M:%S")}
     if use_cache:
         cache[key] = result
         save_cache(cache)
     return dict(result, cached=False)
+
+
+def _tune_fused_graph(ctx, core6, dt, ra, rb, cands, reps: int = 2):
+    """The fused kernel's bands timed as it runs in graph replays (which the
+    eager per-dispatch timer does not reproduce): every uniform height, then
+    split bands around the best one, pairs of about twice its rows split at a
+    few shares for the first blocks dispatched (the split changes only the
+    fused kernel).  Applies and returns (long height, short height or 0,
+    table of microseconds per launch)."""
+    def fused_us(r, r2):
+        ctx.set_tuning(ra, rb, r)
+        ctx.set_band_split(r2)
+        best = float("inf")
+        for _ in range(reps):
+            p = ctx.profile_graph(core6, 8, 1, dt)
+            best = min(best, p["fused"][0] / p["fused"][1] * 1e3)
+        return best
+
+    table = {f"{r}/0": fused_us(r, 0) for r in cands}
+    best = min((t, int(k.split("/")[0]), 0) for k, t in table.items())
+    rf = best[1]
+    shares = SPLIT_SHARES[str(ctx.dtype).replace("torch.", "")]
+    for span in sorted({2 * rf - 4, 2 * rf, 2 * rf + 4}):
+        for share in shares:
+            lng = int(round(span * share))
+            sh = span - lng
+            if lng >= ctx.nx or sh < 4 or f"{lng}/{sh}" in table:
+                continue
+            t = fused_us(lng, sh)
+            table[f"{lng}/{sh}"] = t
+            best = min(best, (t, lng, sh))
+    ctx.set_tuning(ra, rb, best[1])
+    ctx.set_band_split(best[2])
+    return best[1], best[2], {k: round(v, 2) for k, v in table.items()}
 
 
 def tune_batch(batch, members, candidates: Iterable[int] = CANDIDATES, substeps: int = 4,
@@ -117,6 +161,7 @@ def tune_batch(batch, members, candidates: Iterable[int] = CANDIDATES, substeps:
         for m in members:
             m.nbb = substeps
             m.out = None  # the timed runs write their own scratch outputs
+            m.ctx.set_band_split(0)  # batched launches use uniform bands
         for r in candidates:
             for fused in (False, True):
                 for m in members:

@@ -79,6 +79,7 @@ class SWContext:
     precision.  Launches go to the current torch stream of the call."""
 
     tuning = (32, 32, 32)  # rows_a, rows_b, rows_f (read from the library after sw_create)
+    split = 0              # short band height of the fused kernel's split bands (0: uniform)
 
     def __init__(self, consts: Sequence, dtype: torch.dtype = torch.float64):
         if consts is None:  # filled in by like()
@@ -103,6 +104,7 @@ class SWContext:
         rows = [ctypes.c_int() for _ in range(3)]
         _hip.check(_hip.lib().sw_get_tuning(handle, *(ctypes.byref(r) for r in rows)), "sw_get_tuning")
         self.tuning = tuple(r.value for r in rows)
+        self.split = self.band_split()
 
     def like(self) -> "SWContext":
         """Another context of this resolution and precision with its own
@@ -110,6 +112,7 @@ class SWContext:
         other = SWContext(None)
         other.nx, other.N, other.dtype, other.device = self.nx, self.N, self.dtype, self.device
         other.tuning = self.tuning  # sw_create_like copies the band heights
+        other.split = self.split
         handle = ctypes.c_void_p()
         _hip.check(_hip.lib().sw_create_like(ctypes.byref(handle), self._h), "sw_create_like")
         other._h = handle
@@ -181,8 +184,9 @@ class SWContext:
     def profile_graph(self, core6: Sequence[torch.Tensor], pairs: int, replays: int, dtb: float):
         """The fused kernel as it runs in the recorded substep-pair graphs: a
         graph of `pairs` RK2 substep pairs with the kernel's launch stamps on,
-        replayed `replays` times (after one eager substep); returns
-        {"fused": (total ms, launches), "graph": (total ms, replays)}."""
+        `replays` measured replays, each after 3 unmeasured ones queued back
+        to back (after one eager substep); returns {"fused": (total ms,
+        launches), "graph": (total ms, replays)} of the measured replays."""
         ins = [self._check_field(x, STATE_KEYS[k]) for k, x in enumerate(core6)]
         outs = self.empty(3)
         self._bind_stream()
@@ -214,6 +218,18 @@ class SWContext:
         rows_f = self.tuning[2] if rows_f is None else int(rows_f)
         _hip.check(_hip.lib().sw_set_tuning(self._h, int(rows_a), int(rows_b), rows_f), "sw_set_tuning")
         self.tuning = (int(rows_a), int(rows_b), rows_f)
+
+    def set_band_split(self, rows_short: int) -> None:
+        """Split bands of the fused kernel: columns alternate bands of rows_f
+        rows (the first blocks dispatched) and `rows_short` rows (the later
+        ones); 0 restores uniform bands.  Performance only."""
+        _hip.check(_hip.lib().sw_set_band_split(self._h, int(rows_short)), "sw_set_band_split")
+        self.split = int(rows_short)
+
+    def band_split(self) -> int:
+        r = ctypes.c_int()
+        _hip.check(_hip.lib().sw_get_band_split(self._h, ctypes.byref(r)), "sw_get_band_split")
+        return r.value
 
     def layout_flags(self) -> dict:
         f = ctypes.c_int()

@@ -415,6 +415,11 @@ struct MarchArgs {
   T dt, beta;
   int last;
   int S, R, nbands, nwaves;
+  // fused kernel, split bands (R2 > 0): the first `nlong` hardware blocks
+  // (multiple of 8, at most one per CU, so their waves are the oldest on
+  // their SIMDs) march `nwlong` bands of R rows, the rest `nwaves` bands of
+  // R2 rows; columns alternate R and R2 bands.  R2 = 0: uniform R bands.
+  int R2, nlong, nwlong;
   int shared;  // rdx, rdy, rda, c_dy, d_dx identical on every face
   int gsym;    // a_gct symmetric with equal diagonal
   T* sink;     // 256 elements absorbing the stores of inactive lanes
@@ -442,6 +447,7 @@ __device__ __forceinline__ T* at32(T* base, uint32_t i) {
 
 struct WaveTile {
   int t, lane, wslot, c0, j, jl, r0, r1;
+  bool up;  // fused march direction (alternates between neighbouring bands)
   uint32_t fo, mo;  // element offsets (fields stay below 2^29 elements)
 };
 
@@ -529,10 +535,12 @@ __device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRa
 // velocity and flux of the i-face off the edge rows and of the j-face in
 // strips without an edge column, the depth where it is uniform (the scalar
 // argument holds it).  hl: load the halo column (kernel A; the fused kernel
-// has none and passes false, leaving p.hr and p.hh unset).
+// has none and passes false, leaving p.hr and p.hh unset).  je: load the
+// j-face's cube-edge values (the fused kernel passes false and reads them
+// from the band's copy in LDS, leaving p.evj and p.efj zero).
 template <typename T, bool GSYM, bool SAMEP>
 __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w, int i, int crow, int ii,
-                                        int hrow, int hcol, APre<T>& p, bool hl = true) {
+                                        int hrow, int hcol, APre<T>& p, bool hl = true, bool je = true) {
   const int N = a.g.N, nx = a.g.nx;
   const Planes<T>& P = a.P;
   const uint32_t NS = static_cast<uint32_t>(N);
@@ -561,7 +569,7 @@ __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w
   p.ddx = ldg(P.ddx, w.mo + om);
   if constexpr (!SAMEP) p.hp = ldg(a.hp, w.fo + om);
   p.evj = p.efj = T(0);
-  if ((w.c0 <= HALO && HALO <= w.c0 + 63) || (w.c0 <= N - HALO && N - HALO <= w.c0 + 63)) {
+  if (je && ((w.c0 <= HALO && HALO <= w.c0 + 63) || (w.c0 <= N - HALO && N - HALO <= w.c0 + 63))) {
     const bool jedge = (j == HALO || j == N - HALO) && ii >= HALO && ii < N - HALO;
     const int je = jedge ? (t * 4 + (j == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
     p.evj = ldg(a.evel, je);
@@ -1048,6 +1056,7 @@ struct FRing {
   T h[kRing][kRowBuf];
   T b[kRowBuf];         // blended new height of row ia (own columns only)
   T q[kFifo][kFTerms][64];
+  T e[2][2][64];        // cube-edge velocity / flux of the W (0) and E (1) j-faces, rows eb .. eb + 63
 };
 
 // Kernel A's mass flux through a face (barotr_pure.py:29-49) together with
@@ -1085,8 +1094,29 @@ __device__ __forceinline__ bool decode_fwave(const MarchArgs<T>& a, int L, WaveT
   const int N = a.g.N;
   w.lane = threadIdx.x & 63;
   w.wslot = threadIdx.x >> 6;
-  const int wid = L * 4 + w.wslot;
-  if (wid >= a.nwaves) return false;
+  int wid, span, off, height, nw;
+  if (a.R2 > 0) {
+    // the long set is blocks [0, nlong), the short set the rest; each set in
+    // its own XCD-contiguous order (block b runs on XCD b % 8; nlong is a
+    // multiple of 8)
+    const int b = static_cast<int>(blockIdx.x), G = static_cast<int>(gridDim.x);
+    const bool lng = b < a.nlong;
+    const int bb = lng ? b : b - a.nlong, cnt = lng ? a.nlong : G - a.nlong;
+    const int x = bb % kXcds, q = cnt / kXcds, r = cnt % kXcds;
+    wid = (x * q + min(x, r) + bb / kXcds) * 4 + w.wslot;
+    span = a.R + a.R2;
+    off = lng ? 0 : a.R;
+    height = lng ? a.R : a.R2;
+    nw = lng ? a.nwlong : a.nwaves;
+    w.up = !lng;
+  } else {
+    wid = L * 4 + w.wslot;
+    span = height = a.R;
+    off = 0;
+    nw = a.nwaves;
+    w.up = false;
+  }
+  if (wid >= nw) return false;
   w.t = wid % 6;
   const int rest = wid / 6;
   const int s = rest % a.S;
@@ -1094,8 +1124,9 @@ __device__ __forceinline__ bool decode_fwave(const MarchArgs<T>& a, int L, WaveT
   w.c0 = kFStride * s - HALO;
   w.j = w.c0 + w.lane;
   w.jl = min(max(w.j, 0), N - 1);
-  w.r0 = HALO + band * a.R;
-  w.r1 = min(w.r0 + a.R, N - HALO);
+  w.r0 = HALO + band * span + off;
+  w.r1 = min(w.r0 + height, N - HALO);
+  if (a.R2 <= 0) w.up = (band & 1) != 0;
   const uint32_t NN = static_cast<uint32_t>(N) * N;
   w.fo = static_cast<uint32_t>(w.t) * NN;
   w.mo = a.shared ? 0 : w.fo;
@@ -1130,12 +1161,33 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   const int a0 = max(w.r0 - 3, HALO), a1 = min(w.r1 + 3, N - HALO);
   const int b0 = max(w.r0, 2 * HALO), b1 = min(w.r1, N - 2 * HALO);
   T* sink = a.sink + threadIdx.x;
+  // j-faces on cube edges: lane j = HALO (W) or N - HALO (E) needs the
+  // averaged edge velocity / flux of its cell row.  The band's values sit in
+  // LDS, 64 rows from eb, lane k loading row eb + k: one wave-wide load per
+  // 64 rows instead of two per row.
+  const bool ew = w.c0 <= HALO && HALO <= w.c0 + 63, ee = w.c0 <= N - HALO && N - HALO <= w.c0 + 63;
+  const int es = j == HALO ? 0 : 1;  // the side lane j reads (its value is used only on an edge lane)
+  int eb = -1000;
+  auto edge_rows = [&](int base) {
+    const int nx = a.g.nx;
+    const int row = min(base + lane, N - HALO - 1);
+    const int o = row - HALO;
+    if (ew) {
+      rg.e[0][0][lane] = ldg(a.evel, (w.t * 4 + DW) * nx + o);
+      rg.e[0][1][lane] = ldg(a.eflux, (w.t * 4 + DW) * nx + o);
+    }
+    if (ee) {
+      rg.e[1][0][lane] = ldg(a.evel, (w.t * 4 + DE) * nx + o);
+      rg.e[1][1][lane] = ldg(a.eflux, (w.t * 4 + DE) * nx + o);
+    }
+    eb = base;
+  };
 
   // the step's inputs at face i: the newest row's column point, i-face i,
   // cell row ia (no halo column: hl = false)
   auto fetch = [&](int i, APre<T>& p) {
     const int rn = UP ? max(i - 3, 0) : min(i + 2, N - 1);
-    a_fetch<T, GSYM, SAMEP>(a, w, i, rn, UP ? min(i, N - 1) : max(i - 1, 0), rn, w.jl, p, false);
+    a_fetch<T, GSYM, SAMEP>(a, w, i, rn, UP ? min(i, N - 1) : max(i - 1, 0), rn, w.jl, p, false, false);
   };
   auto shift = [](T* x) {
     if (UP) {
@@ -1206,6 +1258,16 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     const int ib = UP ? ia + 3 : ia - 3;
     const int sa = ia & (kRing - 1);
     const uint32_t om = static_cast<uint32_t>(ia) * NS + w.jl;
+    T evj = T(0), efj = T(0);
+    if (ew || ee) {  // wave-uniform: strips holding a cube-edge column
+      if (ia < eb || ia >= eb + 64) {
+        wave_lds_sync();  // earlier reads of the previous rows are done
+        edge_rows(UP ? max(ia - 63, HALO) : ia);
+        wave_lds_sync();
+      }
+      evj = rg.e[es][0][ia - eb];
+      efj = rg.e[es][1][ia - eb];
+    }
     // cell inputs of the velocity terms at row ia
     const T rdx = ldg(P.rdx, w.mo + om), rdy = ldg(P.rdy, w.mo + om), af = ldg(P.af, w.fo + om);
     T upl = T(0), vpl = T(0);
@@ -1239,8 +1301,8 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     // j-face at column j, row ia
     const T vct_c = sv[3], ucov_c = su[3];
     const bool jedge = (j == HALO) || (j == N - HALO);
-    const FaceQ<T> qj = face_q(sh, sv, su, jedge, cur.evj, cur.dzy);
-    const T G = (a.last && jedge) ? cur.efj : qj.F;
+    const FaceQ<T> qj = face_q(sh, sv, su, jedge, evj, cur.dzy);
+    const T G = (a.last && jedge) ? efj : qj.F;
 
     // the cell's bottom (row ia+1) and top (row ia) i-faces
     const T Fb = UP ? Fprev : F, Ft = UP ? F : Fprev;
@@ -1364,7 +1426,7 @@ __device__ __forceinline__ void fused_body(const MarchArgs<T>& a, int L, FRing<T
     return;
   }
   FRing<T>& rg = rings[w.wslot];
-  if ((w.r0 - HALO) / a.R & 1)
+  if (w.up)
     fused_march<T, GSYM, SAMEP, true>(a, w, rg);
   else
     fused_march<T, GSYM, SAMEP, false>(a, w, rg);
@@ -1372,7 +1434,7 @@ __device__ __forceinline__ void fused_body(const MarchArgs<T>& a, int L, FRing<T
 }
 
 template <typename T, bool GSYM, bool SAMEP>
-__global__ __launch_bounds__(256) void k_fused(MarchArgs<T> a) {
+__global__ __launch_bounds__(256, 2) void k_fused(MarchArgs<T> a) {
   __shared__ FRing<T> rings[4];
   fused_body<T, GSYM, SAMEP>(a, xcd_block(), rings);
 }
@@ -1834,7 +1896,7 @@ __global__ __launch_bounds__(256) void k_march_b_batch(BatchLaunch<T> b) {
 }
 
 template <typename T, bool GSYM, bool SAMEP>
-__global__ __launch_bounds__(256) void k_fused_batch(BatchLaunch<T> b) {
+__global__ __launch_bounds__(256, 2) void k_fused_batch(BatchLaunch<T> b) {
   __shared__ FRing<T> rings[4];
   const int L = xcd_block();
   const int m = b.blk2m[L];
@@ -2245,6 +2307,7 @@ struct sw_ctx {
   int rows_a = 32;         // rows per wave of k_march_a
   int rows_b = 32;         // rows per wave of k_march_b
   int rows_f = 32;         // rows per wave of k_fused (sw_create sets default_rows_f); 0 selects k_march_a + k_march_b
+  int rows_f2 = 0;         // > 0: split bands, the short height (the long one is rows_f)
   hipStream_t capture = nullptr;  // private stream used only to record graphs
   struct GraphEntry {
     int rk, pairs;
@@ -2442,7 +2505,7 @@ int edge_pairs(int ue[12]) {
 template <typename T>
 int stage_args(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, const T* up,
                const T* vp, T* h_out, T* u_out, T* v_out, double dt, double beta, int last, int st,
-               int rk, StageArgs<T>& A) {
+               int rk, StageArgs<T>& A, bool split = true) {
   const Geo g = geo_of(c);
   const Planes<T> P = planes_of<T>(c);
   T* evel = edge_buf<T>(c, 2 * st);
@@ -2480,13 +2543,32 @@ int stage_args(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v,
   A.mb.nwaves = 6 * ma.S * A.mb.nbands;
   A.mf = ma;
   A.mf.S = (c->nx + kFStride - 1) / kFStride;
-  A.mf.R = c->rows_f > 0 ? c->rows_f : 32;
+  // without the split (batched runs) a split context's pair of bands counts
+  // as two uniform ones
+  A.mf.R = c->rows_f <= 0 ? 32 : (!split && c->rows_f2 > 0) ? (c->rows_f + c->rows_f2) / 2 : c->rows_f;
   A.mf.nbands = (c->nx + A.mf.R - 1) / A.mf.R;
   A.mf.nwaves = 6 * A.mf.S * A.mf.nbands;
+  A.mf.R2 = A.mf.nlong = A.mf.nwlong = 0;
+  if (split && c->rows_f > 0 && c->rows_f2 > 0 && c->rows_f < c->nx) {
+    const int span = c->rows_f + c->rows_f2;
+    const int pairs = (c->nx + span - 1) / span;                   // long bands per column
+    const int nshort = (c->nx - c->rows_f + span - 1) / span;      // non-empty short bands per column
+    A.mf.R2 = c->rows_f2;
+    A.mf.nwlong = 6 * A.mf.S * pairs;
+    A.mf.nlong = ((A.mf.nwlong + 3) / 4 + kXcds - 1) / kXcds * kXcds;
+    A.mf.nwaves = 6 * A.mf.S * nshort;
+  }
   const uint32_t NN = static_cast<uint32_t>(c->N) * static_cast<uint32_t>(c->N);
   A.hh = HaloArgs<T>{plan_of<T>(c), NN, h_out, 1, nullptr, nullptr};
   A.huv = HaloArgs<T>{plan_of<T>(c), NN, nullptr, 1, u_out, v_out};
   return SW_OK;
+}
+
+// Blocks of a fused-kernel launch: the long set then the short one when the
+// bands are split, else one per four waves.
+template <typename T>
+int fused_blocks(const MarchArgs<T>& m) {
+  return m.R2 > 0 ? m.nlong + (m.nwaves + 3) / 4 : (m.nwaves + 3) / 4;
 }
 
 // One RK stage (stage index st of an rk-stage substep) from (h, hp, u, v, up,
@@ -2510,7 +2592,7 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
     // fused height + deep-cell velocity, new-height halo, frame velocity
     const bool sf = A.mf.hp == A.mf.h && A.mf.up == A.mf.u && A.mf.vp == A.mf.v;
     const int nft = 6 * frame_tiles(c->nx);
-    const dim3 gf((A.mf.nwaves + 3) / 4), gr(nft + static_cast<int>(blocks_for(c->plan_n)));
+    const dim3 gf(fused_blocks(A.mf)), gr(nft + static_cast<int>(blocks_for(c->plan_n)));
     if (c->stamp_next) {  // launch-duration stamps (sw_profile_graph)
       const int waves = 4 * static_cast<int>(gf.x);
       if (c->stamp_next + 2 * waves > c->stamp_end) return fail(SW_E_STATE, "stamp buffer too small");
@@ -2738,9 +2820,12 @@ int run_rk(sw_ctx* c, const void* const in[6], void* const out[13], int nbb, dou
 // the host reads the stamps: a launch lasts from its first wave's entry to
 // its last wave's exit.  ms[K_FUSED] / counts[K_FUSED] sum these over every
 // replay; *graph_ms sums the replays themselves (events around each graph
-// launch).  The first substep, from the caller's fields into X, runs eagerly
-// and untimed; out[0..2] receive the state after 1 + 2 pairs replays
-// substeps.
+// launch).  Each measured replay follows kLeadReplays unmeasured ones queued
+// back to back.  The first substep, from the caller's fields into X, runs
+// eagerly and untimed; out[0..2] receive the state after
+// 1 + 2 pairs (1 + kLeadReplays) replays substeps.
+constexpr int kLeadReplays = 3;  // unmeasured replays ahead of each measured one
+
 template <typename T>
 int run_profile_graph(sw_ctx* c, const void* const in[6], void* const out[13], int pairs, int replays,
                       double dtb, double* ms, int* counts, double* graph_ms) {
@@ -2763,8 +2848,10 @@ int run_profile_graph(sw_ctx* c, const void* const in[6], void* const out[13], i
   T* Y[3] = {scratch_field<T>(c, 9), scratch_field<T>(c, 10), scratch_field<T>(c, 11)};
   if ((rc = run_substep<T>(c, 2, dtb, p[0], p[1], p[2], p[3], p[4], p[5], X, true))) return rc;
   // stamp slots: 2 per wave of each of the graph's 4 pairs fused launches
-  const size_t waves = 4 * static_cast<size_t>((6 * ((c->nx + kFStride - 1) / kFStride) *
-                                                ((c->nx + c->rows_f - 1) / c->rows_f) + 3) / 4);
+  StageArgs<T> shape;  // launch shape of the graph's fused launches
+  if ((rc = stage_args<T>(c, X[0], X[0], X[1], X[2], X[1], X[2], Y[0], Y[1], Y[2], dtb, 0.0, 0, 0, 2, shape)))
+    return rc;
+  const size_t waves = 4 * static_cast<size_t>(fused_blocks(shape.mf));
   const size_t nslots = 2 * waves * 4 * static_cast<size_t>(pairs);
   unsigned long long* dstamps = nullptr;
   SW_HIP(hipMalloc(&dstamps, nslots * sizeof(unsigned long long)));
@@ -2799,7 +2886,12 @@ int run_profile_graph(sw_ctx* c, const void* const in[6], void* const out[13], i
   if (!rc && (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess))
     rc = fail(SW_E_HIP, "hipEventCreate failed");
   for (int r = 0; r < replays && !rc; ++r) {
-    hipError_t e = hipEventRecord(ea, c->stream);
+    // kLeadReplays replays back to back before the measured one, so the
+    // device runs it at the clocks of a sustained sequence of replays (the
+    // timed steps of a benchmark), not those of a device that just idled
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < kLeadReplays && e == hipSuccess; ++q) e = hipGraphLaunch(exec, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
     if (e == hipSuccess) e = hipGraphLaunch(exec, c->stream);
     if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -3134,16 +3226,16 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
       auto at = [&](int st) -> StageArgs<T>& { return row[(static_cast<size_t>(st) * 4 + e) * K + q]; };
       if (rk == 2) {
         rc = stage_args<T>(c, src[0], src[1], src[2], src[3], src[4], src[5], S1[0], S1[1], S1[2],
-                           dtb / 2.0, 0.0, 0, 0, 2, at(0));
+                           dtb / 2.0, 0.0, 0, 0, 2, at(0), false);
         if (!rc) rc = stage_args<T>(c, S1[0], src[1], S1[1], S1[2], src[4], src[5], dst[0], dst[1],
-                                    dst[2], dtb, 0.0, 1, 1, 2, at(1));
+                                    dst[2], dtb, 0.0, 1, 1, 2, at(1), false);
       } else {
         rc = stage_args<T>(c, src[0], src[1], src[2], src[3], src[4], src[5], S1[0], S1[1], S1[2],
-                           dtb / 3.0, 1.0, 0, 0, 3, at(0));
+                           dtb / 3.0, 1.0, 0, 0, 3, at(0), false);
         if (!rc) rc = stage_args<T>(c, S1[0], src[1], S1[1], S1[2], src[4], src[5], S2[0], S2[1],
-                                    S2[2], dtb / 2.0, 1.0, 0, 1, 3, at(1));
+                                    S2[2], dtb / 2.0, 1.0, 0, 1, 3, at(1), false);
         if (!rc) rc = stage_args<T>(c, S2[0], src[1], S2[1], S2[2], src[4], src[5], dst[0], dst[1],
-                                    dst[2], dtb, 1.0, 1, 2, 3, at(2));
+                                    dst[2], dtb, 1.0, 1, 2, 3, at(2), false);
       }
       if (rc) return rc;
     }
@@ -3393,6 +3485,25 @@ int default_rows_f(int nx, int cus, int dtype) {
   return best;
 }
 
+// Default split of the fused kernel's bands (sw_set_band_split) for the
+// uniform height *rows_f: when a launch of uniform bands is one round of
+// resident blocks with more blocks than CUs, the first block on each CU
+// finishes far ahead of the others, so pairs of 2 rows_f rows are split into
+// a long band (69 % in FP64, 60 % in FP32: the measured optima at C384 and
+// C768) for the first blocks and a short one for the rest.  Returns the
+// short height (0: uniform) and sets *rows_f to the long one.
+int default_rows_f2(int nx, int cus, int dtype, int* rows_f) {
+  const long S = (nx + kFStride - 1) / kFStride;
+  const long blocks = (6 * S * ((nx + *rows_f - 1) / *rows_f) + 3) / 4;
+  const long per_cu = dtype == SW_F64 ? 2 : 3;  // resident 4-wave blocks per CU
+  if (cus <= 0 || blocks <= cus || blocks > per_cu * cus) return 0;
+  const int span = 2 * *rows_f;
+  const int lng = static_cast<int>(std::lround(span * (dtype == SW_F64 ? 0.69 : 0.60)));
+  if (lng < 4 || span - lng < 4 || lng >= nx) return 0;
+  *rows_f = lng;
+  return span - lng;
+}
+
 int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CONSTS], void* stream) {
   if (!out) return fail(SW_E_ARG, "out is null");
   *out = nullptr;
@@ -3412,6 +3523,7 @@ int sw_create(sw_ctx** out, int nx, int dtype, const void* const consts[SW_N_CON
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 0;
     c->rows_f = default_rows_f(nx, cus, dtype);
+    c->rows_f2 = default_rows_f2(nx, cus, dtype, &c->rows_f);
     c->device = dev;
   }
   // this device's address of the no-allocation workspace (once per device)
@@ -3460,6 +3572,7 @@ int sw_create_like(sw_ctx** out, const sw_ctx* base) {
   c->rows_a = base->rows_a;
   c->rows_b = base->rows_b;
   c->rows_f = base->rows_f;
+  c->rows_f2 = base->rows_f2;
   int rc = c->dtype == SW_F64 ? alloc_workspace<double>(c) : alloc_workspace<float>(c);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(c->stream);
@@ -3633,6 +3746,21 @@ int sw_set_tuning(sw_ctx* ctx, int rows_a, int rows_b, int rows_f) {
   ctx->rows_b = rows_b;
   ctx->rows_f = rows_f;
   return drop_graphs(ctx);
+}
+
+int sw_set_band_split(sw_ctx* ctx, int rows_short) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  if (rows_short != 0 && (rows_short < 4 || rows_short > 4096))
+    return fail(SW_E_ARG, "rows_short must be 0 or in [4, 4096]");
+  if (rows_short == ctx->rows_f2) return SW_OK;
+  ctx->rows_f2 = rows_short;
+  return drop_graphs(ctx);
+}
+
+int sw_get_band_split(const sw_ctx* ctx, int* rows_short) {
+  if (!ctx || !rows_short) return fail(SW_E_ARG, "null argument");
+  *rows_short = ctx->rows_f2;
+  return SW_OK;
 }
 
 int sw_layout_flags(const sw_ctx* ctx, int* flags) {

@@ -130,9 +130,11 @@ class GridPool:
                 core = [torch.as_tensor(x).to(device=self.device, dtype=dtype)
                         for x in initial.state_tuple(h, u, v)[:6]]
                 r = autotune.tune(ctx, core, initial.courant_dt(g, h, sp))
-                self.tuning[key] = {"rows_a": r["rows_a"], "rows_b": r["rows_b"], "rows_f": r["rows_f"]}
+                self.tuning[key] = {"rows_a": r["rows_a"], "rows_b": r["rows_b"], "rows_f": r["rows_f"],
+                                    "rows_f2": r.get("rows_f2", 0)}
             elif self.rows is not None:
                 ctx.set_tuning(*self.rows)
+                ctx.set_band_split(0)
             self._base[key] = ctx
             return ctx
         return self._base[key].like()
@@ -176,6 +178,7 @@ def main(argv=None) -> int:
         t = pool.tuning.get((m.ctx.nx, tdt))
         if t:
             m.ctx.set_tuning(t["rows_a"], t["rows_b"], t["rows_f"])
+            m.ctx.set_band_split(t.get("rows_f2", 0))
     for m in members:
         m.ctx.rk(m.core6, 2, m.dtb, diagnostics=False, graph=True)  # record graphs
     torch.cuda.synchronize()

@@ -52,10 +52,13 @@ def _run(args) -> int:
     ctx = barotr.context_for(consts, tdt)
     if args.rows:
         ctx.set_tuning(*args.rows)
+    if args.split is not None:
+        ctx.set_band_split(args.split)
     t0 = time.perf_counter()
-    out = ctx.rk(state[:6], args.substeps, dt, graph=False)
+    out = ctx.rk(state[:6], args.substeps, dt, graph=args.graph)
     torch.cuda.synchronize()
-    print(f"profiling run: C{args.nx} {args.dtype} {args.substeps} substeps in "
+    print(f"profiling run: C{args.nx} {args.dtype} {args.substeps} substeps (bands {ctx.tuning[2]}/{ctx.split}, "
+          f"graph {args.graph}) in "
           f"{time.perf_counter() - t0:.3f} s; copy {args.copy_mib} MiB", flush=True)
     del out
     return 0
@@ -201,6 +204,8 @@ def main(argv=None) -> int:
     r.add_argument("--substeps", type=int, default=10)
     r.add_argument("--copy-mib", type=int, default=1024)
     r.add_argument("--rows", type=int, nargs=3, default=None, help="rows_a rows_b rows_f (0: separate kernels)")
+    r.add_argument("--split", type=int, default=None, help="short band height of the fused kernel (0: uniform)")
+    r.add_argument("--graph", action="store_true", help="replay recorded substep-pair graphs")
     s = sub.add_parser("summarize")
     s.add_argument("db", nargs="+")
     s.add_argument("--out", default=None)

@@ -168,6 +168,7 @@ class GPUBackend:
         ctx = barotr.SWContext(self.host_consts(nx), dtype)
         if self.rows is not None:
             ctx.set_tuning(*self.rows)
+            ctx.set_band_split(0)  # batched launches use uniform bands
         return ctx
 
     def free_context(self, ctx) -> None:

@@ -72,15 +72,18 @@ def test_c768_graph_equals_eager_and_band_height_is_invisible(c768):
     import torch
     ctx, core, dt = c768
     ctx.set_tuning(30, 30, 32)
+    ctx.set_band_split(0)
     eager = [x.clone() for x in ctx.rk(core, SUBSTEPS, dt, graph=False)]
     graph = ctx.rk(core, SUBSTEPS, dt, graph=True)
     for k in range(13):
         assert torch.equal(eager[k], graph[k]), k
-    for rows in ((16, 44, 0), (30, 30, 20), (30, 30, 36), (30, 30, 48)):
+    for rows, split in (((16, 44, 0), 0), ((30, 30, 20), 0), ((30, 30, 36), 0), ((30, 30, 48), 0),
+                        ((30, 30, 48), 22), ((30, 30, 50), 22), ((30, 30, 40), 30)):
         ctx.set_tuning(*rows)
+        ctx.set_band_split(split)
         other = ctx.rk(core, SUBSTEPS, dt, graph=True)
         for k in range(3):
-            assert torch.equal(eager[k], other[k]), (rows, k)
+            assert torch.equal(eager[k], other[k]), (rows, split, k)
     assert bool(torch.isfinite(eager[0][:, 3:-3, 3:-3]).all())
 
 

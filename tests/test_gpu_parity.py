@@ -244,6 +244,31 @@ def test_band_height_does_not_change_results(rows):
         _assert_exact(f"rows={rows} {name}", _host(gv), wv)
 
 
+@pytest.mark.parametrize("nx,rows_f,split", [(70, 8, 4), (70, 12, 5), (70, 40, 30), (70, 64, 8), (70, 4, 4),
+                                             (140, 66, 66), (140, 100, 40), (140, 20, 13), (48, 30, 12)])
+def test_band_split_does_not_change_results(nx, rows_f, split):
+    """Split bands of the fused kernel (the first blocks dispatched march
+    rows_f rows, the rest `split` rows; bands longer than 64 rows reload the
+    cube-edge rows they keep in LDS, marching down and up) give the oracle's
+    bits for a stage and for 3 RK2 substeps in graph replays."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(nx)
+    ctx = barotr.SWContext(consts, torch.float64)
+    ctx.set_tuning(32, 32, rows_f)
+    ctx.set_band_split(split)
+    assert ctx.band_split() == split
+    core = random_core(nx, seed=5)
+    want = B.step_rk_logic(*core, consts, 20.0, 0.0, True)
+    got = ctx.stage(*[_dev(x, torch) for x in core], 20.0, 0.0, True)
+    for name, gv, wv in zip(NAMES13, got, want):
+        _assert_exact(f"split {rows_f}/{split} {name}", _host(gv), wv)
+    got = ctx.rk([_dev(x, torch) for x in state[:6]], 3, dt, graph=True)
+    want = B.barotr_rk2_core(state, consts, 3, dt)
+    for k, idx in enumerate((0, 2, 3)):
+        _assert_exact(f"split {rows_f}/{split} rk2 field {idx}", _host(got[k]), want[idx])
+
+
 def test_default_tuning_read_back():
     """sw_create picks the fused kernel's band height from the resolution and
     the device; the context reports what the library holds."""
@@ -253,7 +278,7 @@ def test_default_tuning_read_back():
         _, consts = grid_consts(nx)
         ctx = barotr.SWContext(consts, dtype)
         ra, rb, rf = ctx.tuning
-        assert (ra, rb) == (32, 32) and rf in (8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64)
+        assert (ra, rb) == (32, 32) and (rf in (8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64) or ctx.split > 0)
         ctx.set_tuning(16, 20, 0)
         assert ctx.tuning == (16, 20, 0)
         assert ctx.like().tuning == (16, 20, 0)

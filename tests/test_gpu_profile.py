@@ -27,7 +27,7 @@ def test_profile_graph_outputs_and_counts(dtype):
     pairs, replays = 3, 2
     prof = ctx.profile_graph(core, pairs, replays, dt)
     got = ctx.last_profile_out
-    want = ctx.rk(core, 1 + 2 * pairs * replays, dt, diagnostics=False, graph=False)
+    want = ctx.rk(core, 1 + 2 * pairs * 4 * replays, dt, diagnostics=False, graph=False)  # 3 lead replays each
     for a, b in zip(got, want):
         assert torch.equal(a, b)
     assert prof["fused"][1] == 2 * 2 * pairs * replays and prof["graph"][1] == replays
