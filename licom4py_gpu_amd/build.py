@@ -11,6 +11,7 @@ import time, so the built ``.so`` files travel with the source tree.
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -65,12 +66,35 @@ def hipcc() -> str:
     return path
 
 
+def scratch_kernels(remarks: str) -> list:
+    """Kernels the compiler's resource report shows with private (scratch)
+    memory per lane: register spills or stack objects, which cost every
+    launch a scratch allocation and slow memory traffic."""
+    out, name = [], None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and int(m.group(1)) > 0 and name:
+            out.append((name, int(m.group(1))))
+    return out
+
+
 def build_hip(verbose: bool = False) -> str:
     src = os.path.join(CSRC, "barotr.hip")
-    cmd = [hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", src, "-o", HIP_SO + ".tmp"]
+    cmd = [hipcc(), *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", f"-I{INCLUDE}", f"-I{CSRC}", src,
+           "-o", HIP_SO + ".tmp"]
     if verbose:
         print(" ".join(cmd))
-    _run(cmd)
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        raise RuntimeError(f"build failed: {' '.join(cmd)}")
+    spills = scratch_kernels(proc.stdout)
+    if spills:
+        os.remove(HIP_SO + ".tmp")
+        raise RuntimeError("kernels using scratch memory: " + ", ".join(f"{n} ({b} B/lane)" for n, b in spills))
     os.replace(HIP_SO + ".tmp", HIP_SO)
     return HIP_SO
 

@@ -1439,24 +1439,6 @@ __global__ __launch_bounds__(256, 2) void k_fused(MarchArgs<T> a) {
   fused_body<T, GSYM, SAMEP>(a, xcd_block(), rings);
 }
 
-// Velocity on the frame cells (interior cells within 3 of a face edge, whose
-// pressure-gradient stencil reaches the new height's halo), one thread per
-// cell: kernel B's expressions, in the same order, on stencils gathered from
-// the stage inputs and the halo-filled new height a.hn.
-template <typename T>
-__device__ __forceinline__ T f_evi(const MarchArgs<T>& a, int t, int i, int j) {
-  const int N = a.g.N;
-  const bool e = (i == HALO || i == N - HALO) && j <= N - 4;
-  return ldg(a.evel, e ? (t * 4 + (i == HALO ? DS : DN)) * a.g.nx + (j - HALO) : 0);
-}
-template <typename T>
-__device__ __forceinline__ T f_evj(const MarchArgs<T>& a, int t, int ii, int j) {
-  const int N = a.g.N;
-  const bool e = (j == HALO || j == N - HALO) && ii >= HALO && ii < N - HALO;
-  return ldg(a.evel, e ? (t * 4 + (j == HALO ? DW : DE)) * a.g.nx + (ii - HALO) : 0);
-}
-
-
 // ---------------------------------------------------------------------------
 // duo-grid halo fill (update_domain pass 1 + cube_rmp, closed index form)
 // ---------------------------------------------------------------------------
@@ -1657,6 +1639,25 @@ __global__ __launch_bounds__(256) void k_halo_uv_edge(HaloArgs<T> huv, EdgeArgs<
 // at j, 3 the j-face at j+1; kernel B's expressions) and even / odd lanes the
 // pressure gradient along i / j, and lane 0 gathers them and completes the
 // update in the reference's order of operations.
+// Value of x in lane Q (0..3) of this lane's quad (lanes 4k .. 4k + 3): a DPP
+// quad permutation, a register move with no LDS round trip.  Every lane of
+// the quad must be active.
+template <int Q>
+__device__ __forceinline__ int quad_bcast_b32(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, Q | (Q << 2) | (Q << 4) | (Q << 6), 0xF, 0xF, false);
+}
+template <int Q, typename T>
+__device__ __forceinline__ T quad_bcast(T x) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = static_cast<uint32_t>(quad_bcast_b32<Q>(static_cast<int>(static_cast<uint32_t>(b))));
+    const uint32_t hi = static_cast<uint32_t>(quad_bcast_b32<Q>(static_cast<int>(static_cast<uint32_t>(b >> 32))));
+    return __builtin_bit_cast(T, (static_cast<uint64_t>(hi) << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, quad_bcast_b32<Q>(__builtin_bit_cast(int, x)));
+  }
+}
+
 constexpr int kTile = 21;                    // cells along a tile's long side
 constexpr int kTilePts = (kTile + 6) * 9;    // points of a tile's region
 constexpr int kFrameLanes = 4;
@@ -1702,8 +1703,13 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
   const bool pi = (kl & 1) == 0;
   const uint32_t oc = static_cast<uint32_t>(ii) * NS + static_cast<uint32_t>(j);
   // global inputs besides the tile, issued before the tile loads
-  const T fev = iface ? f_evi(a, t, fr, j) : f_evj(a, t, ii, fc);
+  const int evi_e = (fr == HALO || fr == N - HALO) && j <= N - 4 ? (t * 4 + (fr == HALO ? DS : DN)) * nx + (j - HALO) : 0;
+  const int evj_e = (fc == HALO || fc == N - HALO) && ii >= HALO && ii < N - HALO
+                        ? (t * 4 + (fc == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
+  const T fev = ldg(a.evel, iface ? evi_e : evj_e);
   const uint32_t fd = static_cast<uint32_t>(fr) * NS + static_cast<uint32_t>(fc);
+  // (choosing among the argument struct's pointers per lane would make the
+  // compiler copy the struct to scratch memory: the loads stay in branches)
   const T fdz = iface ? ldg(P.dzx, P.dz_uniform ? 0 : fo + fd) : ldg(P.dzy, P.dz_uniform ? 0 : fo + fd);
   const T rpg = pi ? ldg(P.rdx, mo + oc) : ldg(P.rdy, mo + oc);
   // cell inputs of lane 0's update (lane kl loads the kl-th group)
@@ -1717,7 +1723,9 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
     cin[0] = ldg(a.up, fo + oc); cin[1] = ldg(a.vp, fo + oc); cin[2] = T(0); cin[3] = T(0);
   }
   for (int e = static_cast<int>(threadIdx.x); e < npts; e += 256) {
-    const int rr = e / rcols, cc = e - rr * rcols;
+    // the region is kTile + 6 or 9 points wide: constant divisors
+    const int rr = rcols == 9 ? e / 9 : rcols == kTile + 6 ? e / (kTile + 6) : e / rcols;
+    const int cc = e - rr * rcols;
     const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
     s[0][e] = ldg(a.u, o);
     s[1][e] = ldg(a.v, o);
@@ -1756,7 +1764,9 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
     const T u = s[0][e], v = s[1][e], sina = s[6][e];
     const T g00 = s[2][e], g01 = s[3][e];
     const T g10 = GSYM ? g01 : s[4][e], g11 = GSYM ? g00 : s[5][e];
-    wf[m] = iface ? contra(g00, g01, u, v, sina) : contra(g10, g11, u, v, sina);
+    // the normal component's coefficients chosen per lane, one expression
+    const T ga = iface ? g00 : g10, gb = iface ? g01 : g11;
+    wf[m] = contra(ga, gb, u, v, sina);
     tf[m] = iface ? v : u;
     hf[m] = s[7][e];
   }
@@ -1774,17 +1784,16 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
   const T h1 = c6(hb[1], hb[2], hb[3], hb[4], hb[5], hb[6]);
   const T pgf = T(-9.80) * (rpg * (h1 - h0));
 
-  // gather in lane 0 of the group
-  const int base = (static_cast<int>(threadIdx.x) & 63) & ~(kFrameLanes - 1);
-  const T ucx_prev = __shfl(uc, base + 1), vcx_prev = __shfl(vt, base + 1), Cx_prev = __shfl(C, base + 1);
-  const T vcy = __shfl(uc, base + 2), ucy = __shfl(vt, base + 2), Cy = __shfl(C, base + 2);
-  const T vcy_n = __shfl(uc, base + 3), ucy_n = __shfl(vt, base + 3), Cy_n = __shfl(C, base + 3);
-  const T pgfv = __shfl(pgf, base + 1);
+  // gather in lane 0 of the group (the group is a quad: DPP moves)
+  const T ucx_prev = quad_bcast<1>(uc), vcx_prev = quad_bcast<1>(vt), Cx_prev = quad_bcast<1>(C);
+  const T vcy = quad_bcast<2>(uc), ucy = quad_bcast<2>(vt), Cy = quad_bcast<2>(C);
+  const T vcy_n = quad_bcast<3>(uc), ucy_n = quad_bcast<3>(vt), Cy_n = quad_bcast<3>(C);
+  const T pgfv = quad_bcast<1>(pgf);
   T cv1[4], cv2[2];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) cv1[m] = __shfl(cin[m], base + 1);
-  cv2[0] = __shfl(cin[0], base + 2);
-  cv2[1] = __shfl(cin[1], base + 2);
+  for (int m = 0; m < 4; ++m) cv1[m] = quad_bcast<1>(cin[m]);
+  cv2[0] = quad_bcast<2>(cin[0]);
+  cv2[1] = quad_bcast<2>(cin[1]);
   if (kl != 0) return;
   const T ucx = uc, vcx = vt, Cx = C, pgfu = pgf;
   const int ec = li * rcols + lj;

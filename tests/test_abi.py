@@ -79,3 +79,12 @@ def test_bench_refuses_more_than_one_gpu():
                          capture_output=True, text=True, timeout=300)
     assert out.returncode == 2
     assert "one GPU" in out.stderr
+
+
+def test_build_flags_kernels_with_scratch():
+    from licom4py_gpu_amd import build
+    rep = ("x.hip:1:1: remark: Function Name: k_one [-Rpass-analysis=kernel-resource-usage]\n"
+           "x.hip:1:1: remark:     ScratchSize [bytes/lane]: 0 [-Rpass-analysis=kernel-resource-usage]\n"
+           "x.hip:2:1: remark: Function Name: k_two [-Rpass-analysis=kernel-resource-usage]\n"
+           "x.hip:2:1: remark:     ScratchSize [bytes/lane]: 408 [-Rpass-analysis=kernel-resource-usage]\n")
+    assert build.scratch_kernels(rep) == [("k_two", 408)]
