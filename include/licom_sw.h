@@ -191,6 +191,18 @@ int sw_get_tuning(const sw_ctx* ctx, int* rows_a, int* rows_b, int* rows_f);
 int sw_set_band_split(sw_ctx* ctx, int rows_short);
 int sw_get_band_split(const sw_ctx* ctx, int* rows_short);
 
+/* Form of the launch that follows the fused kernel in every stage (sw_rk2 /
+ * sw_rk3 / sw_stage on one context).  1: one cube-edge launch evaluates the
+ * frame cells, the velocity halo fill and the next stage's averaged edge
+ * values, a block per segment of a cube edge holding both faces' frame cells
+ * in LDS (the default when every halo cell's interpolation sources lie
+ * within 3 along-edge positions of its own, which sw_create checks); 0: the
+ * frame launch, then a launch for the velocity halo fill and the edge pass.
+ * SW_E_ARG when 1 is asked of a grid that fails the check.  Performance
+ * only: results are bit-identical.  Build-defined. */
+int sw_set_frame_form(sw_ctx* ctx, int cube_edge);
+int sw_get_frame_form(const sw_ctx* ctx, int* cube_edge);
+
 /* Host-only: what sw_create detected in the consts.  Bit 0: dzph_x/dzph_y
  * uniform (read as scalars); bit 1: rdx, rdy, rda, c_dy, d_dx identical on all
  * faces (read from face 0); bit 2: a_gct symmetric with equal diagonal. */

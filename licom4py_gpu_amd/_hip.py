@@ -26,7 +26,7 @@ EXPORTED = (
     "sw_halo_scalar", "sw_halo_vector", "sw_edge_avg", "sw_sync",
     "sw_routing_table", "sw_info", "sw_last_error", "sw_set_tuning", "sw_get_tuning", "sw_layout_flags", "sw_profile", "sw_integrals",
     "sw_create_like", "sw_batch_create", "sw_batch_destroy", "sw_batch_run", "sw_integrals_noalloc",
-    "sw_profile_graph", "sw_set_band_split", "sw_get_band_split",
+    "sw_profile_graph", "sw_set_band_split", "sw_get_band_split", "sw_set_frame_form", "sw_get_frame_form",
 )
 
 
@@ -95,6 +95,10 @@ def lib() -> ctypes.CDLL:
     L.sw_set_band_split.restype = ci
     L.sw_get_band_split.argtypes = [vp, ctypes.POINTER(ci)]
     L.sw_get_band_split.restype = ci
+    L.sw_set_frame_form.argtypes = [vp, ci]
+    L.sw_set_frame_form.restype = ci
+    L.sw_get_frame_form.argtypes = [vp, ctypes.POINTER(ci)]
+    L.sw_get_frame_form.restype = ci
     L.sw_layout_flags.argtypes = [vp, ctypes.POINTER(ci)]
     L.sw_layout_flags.restype = ci
     L.sw_create_like.argtypes = [ctypes.POINTER(vp), vp]

@@ -231,6 +231,18 @@ class SWContext:
         _hip.check(_hip.lib().sw_get_band_split(self._h, ctypes.byref(r)), "sw_get_band_split")
         return r.value
 
+    def set_frame_form(self, cube_edge: bool) -> None:
+        """The launch after the fused kernel: one cube-edge launch for the
+        frame cells, velocity halo and next edge pass (True, the default
+        where the grid allows it), or the frame launch and a halo / edge
+        launch (False).  Performance only."""
+        _hip.check(_hip.lib().sw_set_frame_form(self._h, 1 if cube_edge else 0), "sw_set_frame_form")
+
+    def frame_form(self) -> bool:
+        r = ctypes.c_int()
+        _hip.check(_hip.lib().sw_get_frame_form(self._h, ctypes.byref(r)), "sw_get_frame_form")
+        return bool(r.value)
+
     def layout_flags(self) -> dict:
         f = ctypes.c_int()
         _hip.check(_hip.lib().sw_layout_flags(self._h, ctypes.byref(f)), "sw_layout_flags")

@@ -7,7 +7,7 @@
 // kernel/communication.py:355-570) become plain loads from the neighbouring
 // face's memory.
 //
-// One RK stage (momentum/barotr_pure.py:100-168) runs as three launches:
+// One RK stage (momentum/barotr_pure.py:100-168) runs as two launches:
 //   k_fused         new height on the interior (celerity, C-grid velocity,
 //                   LMARS correction, upwind flux, divergence;
 //                   barotr_pure.py:111-120) and new covariant velocity on the
@@ -15,6 +15,16 @@
 //                   the face (D-grid upwind, vorticity / kinetic-energy
 //                   advection, LMARS dissipation, pressure gradient, Coriolis;
 //                   barotr_pure.py:133-150), one column march for both.
+//   k_frame_edge    one block per segment of a cube edge: velocity on the
+//                   3-cell frame of both faces along the segment, the vector
+//                   halo fill (cube.py:137-153) and the scalar fill of the
+//                   new height (cube.py:130-134) on their halo cells, and the
+//                   next stage's cube-edge pass: C-grid normal velocity
+//                   averaged over the two faces (remap.py:107 ->
+//                   communication.py:485-564) and, for a last stage, the
+//                   averaged mass flux (barotr_pure.py:116-117).
+// With the frame form 0 (sw_set_frame_form), and in batched runs (the *_batch
+// kernels), k_frame_edge is replaced by two launches:
 //   k_frame         velocity on the 3-cell frame along the face edges, with
 //                   the new height's duo-grid halo values formed from the halo
 //                   plan, and the halo fill of the new height itself
@@ -271,17 +281,11 @@ __device__ __forceinline__ int halo_entry(int nx, int N, int t, int row, int col
   return t * (12 * nx + 36) + e;
 }
 
-// The vector fill's value of plan entry q (remap to lat-lon at the sources,
-// interpolation, remap back; cube.py:137-153), as k_halo_vector forms it.
+// The vector fill's value of a halo cell from the values (u0, v0), (u1, v1)
+// at its two sources, R its plan entry's rotation coefficients: remap to
+// lat-lon at the sources, interpolation, remap back (cube.py:137-153).
 template <typename T>
-__device__ __forceinline__ void halo_uv_value(const HaloPlan<T>& plan, const T* u, const T* v, int q, T& uo, T& vo) {
-  const int n = plan.n;
-  const uint32_t s0 = plan.s0[q], s1 = plan.s1[q];
-  const T c0 = plan.c0[q], c1 = plan.c1[q];
-  T R[kPlanRot];
-#pragma unroll
-  for (int k = 0; k < kPlanRot; ++k) R[k] = plan.rot[static_cast<size_t>(k) * n + q];
-  const T u0 = u[s0], v0 = v[s0], u1 = u[s1], v1 = v[s1];
+__device__ __forceinline__ void halo_uv_combine(const T* R, T c0, T c1, T u0, T v0, T u1, T v1, T& uo, T& vo) {
   const T ul0 = R[0] * u0 + R[1] * v0;
   const T vl0 = R[2] * u0 + R[3] * v0;
   const T ul1 = R[4] * u1 + R[5] * v1;
@@ -290,6 +294,18 @@ __device__ __forceinline__ void halo_uv_value(const HaloPlan<T>& plan, const T* 
   const T p1 = c0 * vl0 + c1 * vl1;
   uo = R[8] * p0 + R[9] * p1;
   vo = R[10] * p0 + R[11] * p1;
+}
+
+// The same for plan entry q, its sources read from u, v (k_halo_vector).
+template <typename T>
+__device__ __forceinline__ void halo_uv_value(const HaloPlan<T>& plan, const T* u, const T* v, int q, T& uo, T& vo) {
+  const int n = plan.n;
+  const uint32_t s0 = plan.s0[q], s1 = plan.s1[q];
+  const T c0 = plan.c0[q], c1 = plan.c1[q];
+  T R[kPlanRot];
+#pragma unroll
+  for (int k = 0; k < kPlanRot; ++k) R[k] = plan.rot[static_cast<size_t>(k) * n + q];
+  halo_uv_combine(R, c0, c1, u[s0], v[s0], u[s1], v[s1], uo, vo);
 }
 
 template <typename T>
@@ -1530,6 +1546,7 @@ struct PlanBuildArgs {
   uint32_t *dst, *s0, *s1;
   T *c0, *c1, *rot;
   int n;
+  int* far;  // counts entries whose sources lie more than 3 positions from the cell
 };
 
 template <typename T>
@@ -1560,6 +1577,14 @@ __global__ __launch_bounds__(256) void k_plan_build(PlanBuildArgs<T> a) {
                          P.c10[s1], P.c11[s1], P.l00[o], P.l01[o], P.l10[o], P.l11[o]};
 #pragma unroll
   for (int k = 0; k < kPlanRot; ++k) a.rot[static_cast<size_t>(k) * a.n + q] = r[k];
+  // k_frame_edge holds the neighbour's frame cells within 3 along-edge
+  // positions of a halo cell's own position: the sources, at along-edge
+  // indices L-3 and L-2 (strip_sources), must fall there
+  const bool w_e = sj < HALO || sj >= g.N - HALO;
+  const size_t os = g.at(t, si, sj);
+  const int L = w_e ? P.li[os] : P.lj[os];
+  const int r0 = w_e ? si - HALO : sj - HALO;
+  if (L - 3 < max(r0 - HALO, 0) || L - 2 > min(r0 + HALO, nx - 1)) atomicAdd(a.far, 1);
 }
 
 template <typename T>
@@ -1666,35 +1691,31 @@ __host__ __device__ inline int frame_tiles(int nx) {
   return 2 * ((nx + kTile - 1) / kTile) + 2 * ((nx - 6 + kTile - 1) / kTile);
 }
 
+// A rectangle of frame cells: crows x ccols cells of face t from (cr0, cc0),
+// one side at most kTile long and the other 3.
+struct FTile {
+  int t, cr0, crows, cc0, ccols;
+};
+
+// The frame cells of tile ft, evaluated by a group of 256 threads (ltid its
+// thread index in the group): the group loads the region its cells' stencils
+// read (the cells and 3 points around them) into s, then 4 lanes take each
+// cell.  Holds one __syncthreads(), which every thread of the block reaches.
+// Returns true on the lane that completes a cell, with its new velocity
+// (un, vn), its position (ci, cj) and its tile-local index cq = (ci - cr0) *
+// ccols + (cj - cc0).
 template <typename T, bool GSYM>
-__device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan<T>& plan, int blk,
-                                           T (*s)[kTilePts]) {
+__device__ __forceinline__ bool frame_cell(const MarchArgs<T>& a, const HaloPlan<T>& plan, const FTile& ft,
+                                           T (*s)[kTilePts], int ltid, T& un, T& vn, int& ci, int& cj, int& cq) {
   const int nx = a.g.nx, N = a.g.N;
-  const int nr = (nx + kTile - 1) / kTile, ns = (nx - 6 + kTile - 1) / kTile;
-  const int per = 2 * nr + 2 * ns;
-  const int t = blk / per;
-  int k = blk - t * per, cr0, crows, cc0, ccols;
-  if (k < 2 * nr) {
-    const int g = k / nr, c = k - g * nr;
-    cr0 = g ? N - 2 * HALO : HALO;
-    crows = 3;
-    cc0 = HALO + kTile * c;
-    ccols = min(kTile, nx - kTile * c);
-  } else {
-    k -= 2 * nr;
-    const int g = k / ns, c = k - g * ns;
-    cr0 = 2 * HALO + kTile * c;
-    crows = min(kTile, nx - 6 - kTile * c);
-    cc0 = g ? N - 2 * HALO : HALO;
-    ccols = 3;
-  }
+  const int t = ft.t, cr0 = ft.cr0, crows = ft.crows, cc0 = ft.cc0, ccols = ft.ccols;
   const int ra = cr0 - 3, ca = cc0 - 3, rcols = ccols + 6, npts = (crows + 6) * rcols;
   const Planes<T>& P = a.P;
   const uint32_t NS = static_cast<uint32_t>(N);
   const uint32_t fo = static_cast<uint32_t>(t) * NS * NS, mo = a.shared ? 0 : fo;
   // the cell of this lane's group (the last cell for lanes past the tile's
   // cells, which only help fill the tile)
-  const int q0 = static_cast<int>(threadIdx.x) / kFrameLanes, kl = static_cast<int>(threadIdx.x) % kFrameLanes;
+  const int q0 = ltid / kFrameLanes, kl = ltid % kFrameLanes;
   const int q = min(q0, crows * ccols - 1);
   const int ii = cr0 + q / ccols, j = cc0 + q % ccols;
   const int li = ii - ra, lj = j - ca;
@@ -1722,7 +1743,7 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
   } else {
     cin[0] = ldg(a.up, fo + oc); cin[1] = ldg(a.vp, fo + oc); cin[2] = T(0); cin[3] = T(0);
   }
-  for (int e = static_cast<int>(threadIdx.x); e < npts; e += 256) {
+  for (int e = ltid; e < npts; e += 256) {
     // the region is kTile + 6 or 9 points wide: constant divisors
     const int rr = rcols == 9 ? e / 9 : rcols == kTile + 6 ? e / (kTile + 6) : e / rcols;
     const int cc = e - rr * rcols;
@@ -1750,7 +1771,7 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
     }
   }
   __syncthreads();
-  if (q0 >= crows * ccols) return;  // whole groups: 4 divides 64
+  if (q0 >= crows * ccols) return false;  // whole groups: 4 divides 64
   const T wn = T(1.0) - a.beta, wo = a.beta;
 
   // face kl from its 6 points: normal velocity (contravariant), tangential
@@ -1794,7 +1815,7 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
   for (int m = 0; m < 4; ++m) cv1[m] = quad_bcast<1>(cin[m]);
   cv2[0] = quad_bcast<2>(cin[0]);
   cv2[1] = quad_bcast<2>(cin[1]);
-  if (kl != 0) return;
+  if (kl != 0) return false;
   const T ucx = uc, vcx = vt, Cx = C, pgfu = pgf;
   const int ec = li * rcols + lj;
   const T uc0 = s[0][ec], vc0 = s[1][ec], sc0 = s[6][ec];
@@ -1814,8 +1835,266 @@ __device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan
   const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
   const T visx = -(rdx * (Cx - Cx_prev));
   const T visy = -(rdy * (Cy_n - Cy));
-  *at32(a.u_out, fo + oc) = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
-  *at32(a.v_out, fo + oc) = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
+  un = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
+  vn = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
+  ci = ii;
+  cj = j;
+  cq = q;
+  return true;
+}
+
+// Lane 1's value of x in lane 0 of each lane pair (a DPP quad permutation
+// [1, 1, 3, 3]); both lanes of the pair must be active.
+template <typename T>
+__device__ __forceinline__ T pair_hi(T x) {
+  constexpr int sel = 1 | (1 << 2) | (3 << 4) | (3 << 6);
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b)), sel, 0xF, 0xF, false));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b >> 32)), sel, 0xF, 0xF, false));
+    return __builtin_bit_cast(T, (static_cast<uint64_t>(hi) << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), sel, 0xF, 0xF, false));
+  }
+}
+
+// The other lane's value of x in each lane pair (a DPP quad permutation
+// [1, 0, 3, 2]); both lanes of the pair must be active.
+template <typename T>
+__device__ __forceinline__ T pair_swap(T x) {
+  constexpr int sel = 1 | (0 << 2) | (3 << 4) | (2 << 6);
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b)), sel, 0xF, 0xF, false));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b >> 32)), sel, 0xF, 0xF, false));
+    return __builtin_bit_cast(T, (static_cast<uint64_t>(hi) << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), sel, 0xF, 0xF, false));
+  }
+}
+
+// frame_cell with two lanes per cell, for groups of 128 threads: lane 0 of a
+// pair evaluates the cell's bottom i-face (row ii+1), its j-face at j and the
+// pressure gradient along i, lane 1 the top i-face (row ii), the j-face at
+// j+1 and the gradient along j; lane 0 completes the update.  The same
+// expressions on the same operands as frame_cell.
+constexpr int kFrameGroup2 = 128;
+
+// Planes of frame_cell2's LDS region: u v g00 g01 [g10 g11] sina h hn (the
+// a_gct planes g10, g11 only when the metric is not symmetric).
+template <bool GSYM>
+struct RegionPlanes {
+  static constexpr int u = 0, v = 1, g00 = 2, g01 = 3, g10 = 4, g11 = 5;
+  static constexpr int sina = GSYM ? 4 : 6, h = GSYM ? 5 : 7, hn = GSYM ? 6 : 8, n = GSYM ? 7 : 9;
+};
+
+// `pre` runs after the region loads are issued and before the barrier.
+template <typename T, bool GSYM, typename Pre>
+__device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPlan<T>& plan, const FTile& ft,
+                                            T (*s)[kTilePts], int ltid, T& un, T& vn, int& ci, int& cj, int& cq,
+                                            Pre&& pre) {
+  using RP = RegionPlanes<GSYM>;
+  const int nx = a.g.nx, N = a.g.N;
+  const int t = ft.t, cr0 = ft.cr0, crows = ft.crows, cc0 = ft.cc0, ccols = ft.ccols;
+  const int ra = cr0 - 3, ca = cc0 - 3, rcols = ccols + 6, npts = (crows + 6) * rcols;
+  const Planes<T>& P = a.P;
+  const uint32_t NS = static_cast<uint32_t>(N);
+  const uint32_t fo = static_cast<uint32_t>(t) * NS * NS, mo = a.shared ? 0 : fo;
+  const int q0 = ltid >> 1, kl = ltid & 1;
+  const int q = min(q0, crows * ccols - 1);
+  const int ii = cr0 + q / ccols, j = cc0 + q % ccols;
+  const int li = ii - ra, lj = j - ca;
+  const int fr = kl == 0 ? ii + 1 : ii, fc = kl == 0 ? j : j + 1;
+  const uint32_t oc = static_cast<uint32_t>(ii) * NS + static_cast<uint32_t>(j);
+  // global inputs besides the tile, issued before the tile loads
+  const int evi_e = (fr == HALO || fr == N - HALO) && j <= N - 4 ? (t * 4 + (fr == HALO ? DS : DN)) * nx + (j - HALO) : 0;
+  const int evj_e = (fc == HALO || fc == N - HALO) && ii >= HALO && ii < N - HALO
+                        ? (t * 4 + (fc == HALO ? DW : DE)) * nx + (ii - HALO) : 0;
+  const T fevi = ldg(a.evel, evi_e), fevj = ldg(a.evel, evj_e);
+  const uint32_t fdi = static_cast<uint32_t>(fr) * NS + static_cast<uint32_t>(j);
+  const uint32_t fdj = static_cast<uint32_t>(ii) * NS + static_cast<uint32_t>(fc);
+  const T fdzi = ldg(P.dzx, P.dz_uniform ? 0 : fo + fdi), fdzj = ldg(P.dzy, P.dz_uniform ? 0 : fo + fdj);
+  const T rpg = kl == 0 ? ldg(P.rdx, mo + oc) : ldg(P.rdy, mo + oc);
+  // cell inputs of the update: lane 0 loads its own, lane 1 the rest
+  const uint32_t on = oc + NS;
+  T cin[6];
+  if (kl == 0) {
+    cin[0] = ldg(P.cdy, mo + on); cin[1] = ldg(P.cdy, mo + oc); cin[2] = ldg(P.rda, mo + oc); cin[3] = ldg(P.ddx, mo + oc);
+    cin[4] = T(0); cin[5] = T(0);
+  } else {
+    cin[0] = ldg(P.ddx, mo + oc + 1); cin[1] = ldg(P.rdx, mo + oc); cin[2] = ldg(P.rdy, mo + oc); cin[3] = ldg(P.af, fo + oc);
+    cin[4] = ldg(a.up, fo + oc); cin[5] = ldg(a.vp, fo + oc);
+  }
+  // the region (at most kTilePts = 2 x 128 - 13 points): each thread loads
+  // points ltid and ltid + 128, every load issued before the first LDS write
+  struct Pt {
+    T u, v, g00, g01, g10, g11, sina, h, hn;
+  };
+  auto fetch = [&](int e, Pt& p) {
+    const int rr = rcols == 9 ? e / 9 : rcols == kTile + 6 ? e / (kTile + 6) : e / rcols;
+    const int cc = e - rr * rcols;
+    const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
+    p.u = ldg(a.u, o);
+    p.v = ldg(a.v, o);
+    p.g00 = ldg(P.g00, o);
+    p.g01 = ldg(P.g01, o);
+    if (!GSYM) {
+      p.g10 = ldg(P.g10, o);
+      p.g11 = ldg(P.g11, o);
+    }
+    p.sina = ldg(P.sina, o);
+    p.h = ldg(a.h, o);
+    const int row = ra + rr, col = ca + cc;
+    const bool hrow = row < HALO || row >= N - HALO, hcol = col < HALO || col >= N - HALO;
+    if (hrow != hcol) {
+      const int pq = halo_entry(nx, N, t, row, col);
+      p.hn = plan.c0[pq] * ldg(a.hn, plan.s0[pq]) + plan.c1[pq] * ldg(a.hn, plan.s1[pq]);
+    } else {
+      p.hn = ldg(a.hn, o);
+    }
+  };
+  auto put = [&](int e, const Pt& p) {
+    s[RP::u][e] = p.u;
+    s[RP::v][e] = p.v;
+    s[RP::g00][e] = p.g00;
+    s[RP::g01][e] = p.g01;
+    if (!GSYM) {
+      s[RP::g10][e] = p.g10;
+      s[RP::g11][e] = p.g11;
+    }
+    s[RP::sina][e] = p.sina;
+    s[RP::h][e] = p.h;
+    s[RP::hn][e] = p.hn;
+  };
+  static_assert(kTilePts <= 2 * kFrameGroup2, "two region points per thread");
+  const int e1 = ltid + kFrameGroup2;
+  Pt pt0, pt1;
+  fetch(min(ltid, npts - 1), pt0);
+  fetch(min(e1, npts - 1), pt1);
+  if (ltid < npts) put(ltid, pt0);
+  if (e1 < npts) put(e1, pt1);
+  pre();
+  __syncthreads();
+  if (q0 >= crows * ccols) return false;  // whole pairs
+  const T wn = T(1.0) - a.beta, wo = a.beta;
+
+  // one face from its 6 points (frame_cell's expressions)
+  auto face = [&](bool iface, int r, int c, T fev, T fdz, T& uc, T& vt, T& C) {
+    const int e0 = iface ? (r - 3 - ra) * rcols + lj : li * rcols + (c - 3 - ca);
+    const int es = iface ? rcols : 1;
+    T wf[6], tf[6], hf[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const int e = e0 + m * es;
+      const T u = s[RP::u][e], v = s[RP::v][e], sina = s[RP::sina][e];
+      const T g00 = s[RP::g00][e], g01 = s[RP::g01][e];
+      const T g10 = GSYM ? g01 : s[RP::g10][e], g11 = GSYM ? g00 : s[RP::g11][e];
+      wf[m] = iface ? contra(g00, g01, u, v, sina) : contra(g10, g11, u, v, sina);
+      tf[m] = iface ? v : u;
+      hf[m] = s[RP::h][e];
+    }
+    const bool fedge = iface ? (r == HALO) || (r == N - HALO) : (c == HALO) || (c == N - HALO);
+    iface_b(wf, tf, hf, fedge, fev, fdz, uc, vt, C);
+  };
+  // the i-face, then the j-face: a loop that is not unrolled, so that one
+  // face's operands are live at a time
+  T uci = T(0), vti = T(0), Ci = T(0), ucj = T(0), vtj = T(0), Cj = T(0);
+#pragma unroll 1
+  for (int f = 0; f < 2; ++f) {
+    T uc, vt, C;
+    if (f == 0)
+      face(true, fr, j, fevi, fdzi, uc, vt, C);
+    else
+      face(false, ii, fc, fevj, fdzj, uc, vt, C);
+    if (f == 0) {
+      uci = uc; vti = vt; Ci = C;
+    } else {
+      ucj = uc; vtj = vt; Cj = C;
+    }
+  }
+
+  // pressure gradient along i (lane 0) or j (lane 1) at (ii, j)
+  const bool pi = kl == 0;
+  const int p0 = pi ? (li - 3) * rcols + lj : li * rcols + lj - 3;
+  const int ps = pi ? rcols : 1;
+  T hb[7];
+#pragma unroll
+  for (int m = 0; m < 7; ++m) hb[m] = wn * s[RP::hn][p0 + m * ps] + wo * s[RP::h][p0 + m * ps];
+  const T h0 = c6(hb[0], hb[1], hb[2], hb[3], hb[4], hb[5]);
+  const T h1 = c6(hb[1], hb[2], hb[3], hb[4], hb[5], hb[6]);
+  const T pgf = T(-9.80) * (rpg * (h1 - h0));
+
+  // lane 1's faces (top i-face, j-face at j+1), gradient and inputs
+  const T ucx_prev = pair_hi(uci), vcx_prev = pair_hi(vti), Cx_prev = pair_hi(Ci);
+  const T vcy_n = pair_hi(ucj), ucy_n = pair_hi(vtj), Cy_n = pair_hi(Cj);
+  const T pgfv = pair_hi(pgf);
+  T cv[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) cv[m] = pair_hi(cin[m]);
+  if (kl != 0) return false;
+  const T ucx = uci, vcx = vti, Cx = Ci, pgfu = pgf;
+  const T vcy = ucj, ucy = vtj, Cy = Cj;
+  const int ec = li * rcols + lj;
+  const T uc0 = s[RP::u][ec], vc0 = s[RP::v][ec], sc0 = s[RP::sina][ec];
+  const T g00c = s[RP::g00][ec], g01c = s[RP::g01][ec];
+  const T g10c = GSYM ? g01c : s[RP::g10][ec], g11c = GSYM ? g00c : s[RP::g11][ec];
+  const T vct_c = contra(g10c, g11c, uc0, vc0, sc0);
+  const T uct_c = contra(g00c, g01c, uc0, vc0, sc0);
+  const T cdy_i = cin[0], cdy_prev = cin[1], rda = cin[2], ddx = cin[3];
+  const T ddx_n = cv[0], rdx = cv[1], rdy = cv[2], af = cv[3];
+  const T up = cv[4], vp = cv[5];
+  const T zeta = rda * (((ucy * ddx + vcx * cdy_i) - ucy_n * ddx_n) - vcx_prev * cdy_prev);
+  const T ku1 = T(0.5) * (ucx * ucx + vcx * vcx);
+  const T ku0 = T(0.5) * (ucx_prev * ucx_prev + vcx_prev * vcx_prev);
+  const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
+  const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+  const T advx = zeta * vct_c - (ku1 - ku0) * rdx;
+  const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
+  const T visx = -(rdx * (Cx - Cx_prev));
+  const T visy = -(rdy * (Cy_n - Cy));
+  un = up + (((pgfu + advx) + af * vct_c) + visx) * a.dt;
+  vn = vp + (((pgfv + advy) - af * uct_c) + visy) * a.dt;
+  ci = ii;
+  cj = j;
+  cq = q;
+  return true;
+}
+
+template <typename T, bool GSYM>
+__device__ __forceinline__ void frame_body(const MarchArgs<T>& a, const HaloPlan<T>& plan, int blk,
+                                           T (*s)[kTilePts]) {
+  const int nx = a.g.nx, N = a.g.N;
+  const int nr = (nx + kTile - 1) / kTile, ns = (nx - 6 + kTile - 1) / kTile;
+  const int per = 2 * nr + 2 * ns;
+  FTile ft;
+  ft.t = blk / per;
+  int k = blk - ft.t * per;
+  if (k < 2 * nr) {
+    const int g = k / nr, c = k - g * nr;
+    ft.cr0 = g ? N - 2 * HALO : HALO;
+    ft.crows = 3;
+    ft.cc0 = HALO + kTile * c;
+    ft.ccols = min(kTile, nx - kTile * c);
+  } else {
+    k -= 2 * nr;
+    const int g = k / ns, c = k - g * ns;
+    ft.cr0 = 2 * HALO + kTile * c;
+    ft.crows = min(kTile, nx - 6 - kTile * c);
+    ft.cc0 = g ? N - 2 * HALO : HALO;
+    ft.ccols = 3;
+  }
+  T un, vn;
+  int ci, cj, cq;
+  if (frame_cell<T, GSYM>(a, plan, ft, s, static_cast<int>(threadIdx.x), un, vn, ci, cj, cq)) {
+    const uint32_t o = (static_cast<uint32_t>(ft.t) * static_cast<uint32_t>(N) + static_cast<uint32_t>(ci)) *
+                           static_cast<uint32_t>(N) + static_cast<uint32_t>(cj);
+    *at32(a.u_out, o) = un;
+    *at32(a.v_out, o) = vn;
+  }
 }
 
 // The frame tiles (blocks 0 .. nft-1) and the new height's halo fill (the
@@ -1833,6 +2112,285 @@ template <typename T, bool GSYM>
 __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a, HaloArgs<T> hh, int nft) {
   __shared__ T s[9][kTilePts];  // u v g00 g01 g10 g11 sina h hn
   frame_h_body<T, GSYM>(a, hh, nft, static_cast<int>(blockIdx.x), s);
+}
+
+// ---------------------------------------------------------------------------
+// Cube-edge frame kernel: the frame cells, both halo fills and the next
+// stage's edge pass in one launch.
+//
+// Block b takes cube edge e = b / nseg and the kSeg along-edge positions
+// [p0, p1) of its first face edge (A = (t, d)); the second face edge (B,
+// across the joint) owns the mirrored positions.  Each half of the block
+// (128 threads, two per cell: frame_cell2) evaluates the frame cells of one
+// side over its owned positions widened by 3 on either end and keeps their
+// new velocity in LDS.  A halo cell at an owned position takes its two
+// interpolation sources from the other side's frame cells, at most 3
+// positions away (checked when the plan is built), so the block forms the
+// vector fill of the owned halo cells (cube.py:137-153) from LDS and writes
+// the new height's fill from its LDS region, where frame_cell2 formed it
+// from the plan.  Then it forms the next stage's averaged edge velocity /
+// flux of the owned positions (communication.py:485-564), whose six stencil
+// points across the edge are three frame cells and three halo cells it
+// holds.  The halo corners are filled by the blocks holding positions 0 and
+// nx-1 of the W / E face edges, whose strip cells they copy
+// (cube.py:96-125).  Every value is the expression the separate kernels
+// evaluate, on the same operands.
+// ---------------------------------------------------------------------------
+constexpr int kSeg = kTile - 6;       // owned positions per block (widened: kTile)
+constexpr int kFrameEdgeThreads = 2 * kFrameGroup2;
+
+// One side of a cube-edge block: face edge (t, d), owned positions [lo, hi),
+// evaluated positions [pa, pa + m) and the frame tile holding them.
+struct ESide {
+  int t, d, lo, hi, pa, m;
+  FTile ft;
+};
+
+__device__ __forceinline__ ESide eside(int nx, int N, int t, int d, int lo, int hi) {
+  ESide S;
+  S.t = t;
+  S.d = d;
+  S.lo = lo;
+  S.hi = hi;
+  S.pa = max(lo - HALO, 0);
+  S.m = min(hi + HALO, nx) - S.pa;
+  S.ft.t = t;
+  if (d == DS || d == DN) {
+    S.ft.cr0 = d == DS ? HALO : N - 2 * HALO;
+    S.ft.crows = 3;
+    S.ft.cc0 = HALO + S.pa;
+    S.ft.ccols = S.m;
+  } else {
+    S.ft.cr0 = HALO + S.pa;
+    S.ft.crows = S.m;
+    S.ft.cc0 = d == DW ? HALO : N - 2 * HALO;
+    S.ft.ccols = 3;
+  }
+  return S;
+}
+
+// Tile-local index of cell (i, j) of the side's tile.
+__device__ __forceinline__ int eside_cell(const ESide& S, int i, int j) {
+  return (i - S.ft.cr0) * S.ft.ccols + (j - S.ft.cc0);
+}
+
+// Region-local index (frame_cell's LDS layout) of point (i, j) of the side.
+__device__ __forceinline__ int eside_point(const ESide& S, int i, int j) {
+  return (i - (S.ft.cr0 - 3)) * (S.ft.ccols + 6) + (j - (S.ft.cc0 - 3));
+}
+
+template <typename T, bool GSYM>
+struct FrameEdgeLds {
+  T s[2][RegionPlanes<GSYM>::n][kTilePts];  // the two sides' regions (frame_cell2's planes)
+  T ur[2][3 * kTile];      // new velocity of the evaluated frame cells
+  T vr[2][3 * kTile];
+  T uh[2][3][kSeg];        // halo values at the owned positions, layers 1..3
+  T vh[2][3][kSeg];
+  T rot[kPlanRot][128];    // rotation coefficients of the halo entries of threads 0..127
+};
+
+// Contravariant normal velocity (edge_body's stencil value) and new height of
+// point (i, j) of side X: halo points take the block's fill values, interior
+// points its frame values.
+template <typename T, bool GSYM>
+__device__ __forceinline__ void eside_sample(const FrameEdgeLds<T, GSYM>& L, int X, const ESide& S, int N, int i,
+                                             int j, int pos, T& w, T& h) {
+  using RP = RegionPlanes<GSYM>;
+  const bool ns = S.d == DS || S.d == DN;
+  const int x = ns ? i : j;
+  T u, v;
+  if (x < HALO || x >= N - HALO) {
+    const int layer = x < HALO ? HALO - x : x - (N - HALO - 1);
+    u = L.uh[X][layer - 1][pos - S.lo];
+    v = L.vh[X][layer - 1][pos - S.lo];
+  } else {
+    const int c = eside_cell(S, i, j);
+    u = L.ur[X][c];
+    v = L.vr[X][c];
+  }
+  const int e = eside_point(S, i, j);
+  const T g00 = L.s[X][RP::g00][e], g01 = L.s[X][RP::g01][e], sina = L.s[X][RP::sina][e];
+  const T g10 = GSYM ? g01 : L.s[X][RP::g10][e], g11 = GSYM ? g00 : L.s[X][RP::g11][e];
+  w = ns ? contra(g00, g01, u, v, sina) : contra(g10, g11, u, v, sina);
+  h = L.s[X][RP::hn][e];
+}
+
+// Edge velocity (and the six heights) of side X at position pos: the six
+// stencil points across the edge line (remap.py:42-55 at rows / columns 3,
+// N-3), as edge_body evaluates them.
+template <typename T, bool GSYM>
+__device__ __forceinline__ T eside_vel(const FrameEdgeLds<T, GSYM>& L, int X, const ESide& S, int N, int pos, T* hv) {
+  const bool ns = S.d == DS || S.d == DN;
+  const int line = (S.d == DS || S.d == DW) ? HALO : N - HALO;
+  T w[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int i = ns ? line - 3 + k : HALO + pos, j = ns ? HALO + pos : line - 3 + k;
+    eside_sample<T, GSYM>(L, X, S, N, i, j, pos, w[k], hv[k]);
+  }
+  return T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
+}
+
+// Upwind mass flux of side X through the edge (barotr_pure.py:29-49) with the
+// averaged edge velocity avg, as edge_body evaluates it.
+template <typename T>
+__device__ __forceinline__ T eside_flux(const T* hv, T avg, T dz) {
+  const T he = e5(hv[0], hv[1], hv[2], hv[3], hv[4]);
+  const T hw = w5(hv[1], hv[2], hv[3], hv[4], hv[5]);
+  const T c = celerity(c6(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5]), dz);
+  const T vc = avg + k49_over(c) * (he - hw);
+  return dz * vc + (vc > T(0) ? he : hw) * vc;
+}
+
+// dz of side S's edge-line cell at position pos (edge_body's dz).
+template <typename T>
+__device__ __forceinline__ T eside_dz(const EdgeArgs<T>& en, const ESide& S, int pos) {
+  const int N = en.g.N;
+  const bool ns = S.d == DS || S.d == DN;
+  const int line = (S.d == DS || S.d == DW) ? HALO : N - HALO;
+  const size_t oc = ns ? en.g.at(S.t, line, HALO + pos) : en.g.at(S.t, HALO + pos, line);
+  return ns ? dzx(en.P, oc) : dzy(en.P, oc);
+}
+
+template <typename T, bool GSYM>
+__device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const EdgeArgs<T>& en, int blk,
+                                                FrameEdgeLds<T, GSYM>& L) {
+  using RP = RegionPlanes<GSYM>;
+  const int nx = a.g.nx, N = a.g.N;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int nseg = (nx + kSeg - 1) / kSeg;
+  const int e = blk / nseg, p0 = (blk - e * nseg) * kSeg, p1 = min(p0 + kSeg, nx);
+  const int ta = en.ue[e] >> 2, da = en.ue[e] & 3;
+  const int tb = en.R.r[ta][da][0], db = en.R.r[ta][da][1], flip = en.R.r[ta][da][2];
+  const ESide A = eside(nx, N, ta, da, p0, p1);
+  const ESide B = eside(nx, N, tb, db, flip ? nx - p1 : p0, flip ? nx - p0 : p1);
+  const HaloPlan<T>& plan = en.plan;
+  const uint32_t NS = static_cast<uint32_t>(N), NN = NS * NS;
+
+  // the halo entries of the owned positions and of the corners they carry
+  // (threads 0..127, side tid / 64): their plan data is fetched with the
+  // frame loads, the rotation coefficients staged in LDS
+  const int hs = tid >> 6, hk = tid & 63;
+  const ESide H = hs ? B : A;
+  const int nown = H.hi - H.lo;
+  const bool we = H.d == DW || H.d == DE;
+  const int c_lo = (we && H.lo == 0) ? 1 : 0, c_hi = (we && H.hi == nx) ? 1 : 0;
+  const bool hact = tid < 128 && hk < 3 * nown + 9 * (c_lo + c_hi);
+  int hq = 0, hlayer = 0, hpos = 0, hrow = 0, hcol = 0;
+  if (hact) {
+    if (hk < 3 * nown) {
+      hlayer = hk / nown + 1;
+      hpos = H.lo + hk % nown;
+      hrow = H.d == DS ? HALO - hlayer : H.d == DN ? N - HALO - 1 + hlayer : HALO + hpos;
+      hcol = H.d == DW ? HALO - hlayer : H.d == DE ? N - HALO - 1 + hlayer : HALO + hpos;
+      hq = halo_entry(nx, N, H.t, hrow, hcol);
+    } else {
+      // corners 0 (S W), 1 (S E), 2 (N W), 3 (N E): 9 cells each
+      const int kc = hk - 3 * nown;
+      const bool north = (kc >= 9) || !c_lo;
+      const int corner = (north ? 2 : 0) + (H.d == DE ? 1 : 0);
+      hq = H.t * (12 * nx + 36) + 12 * nx + corner * 9 + (kc % 9);
+    }
+  }
+  uint32_t hdst = 0, hs0 = 0, hs1 = 0;
+  T hc0 = T(0), hc1 = T(0), R[kPlanRot];
+  if (hact) {
+    hdst = plan.dst[hq];
+    hs0 = plan.s0[hq];
+    hs1 = plan.s1[hq];
+    hc0 = plan.c0[hq];
+    hc1 = plan.c1[hq];
+#pragma unroll
+    for (int k = 0; k < kPlanRot; ++k) R[k] = plan.rot[static_cast<size_t>(k) * plan.n + hq];
+  }
+  auto stage_rot = [&]() {
+    if (hact) {
+#pragma unroll
+      for (int k = 0; k < kPlanRot; ++k) L.rot[k][tid] = R[k];
+    }
+  };
+
+  // frame cells of both sides
+  {
+    const int side = tid >= kFrameGroup2 ? 1 : 0;
+    const ESide S = side ? B : A;
+    T un, vn;
+    int ci, cj, cq;
+    if (frame_cell2<T, GSYM>(a, plan, S.ft, L.s[side], tid - side * kFrameGroup2, un, vn, ci, cj, cq, stage_rot)) {
+      L.ur[side][cq] = un;
+      L.vr[side][cq] = vn;
+      const bool ns = S.d == DS || S.d == DN;
+      const int pos = ns ? cj - HALO : ci - HALO;
+      // W / E face edges leave the corner frame blocks to the S / N ones
+      if (pos >= S.lo && pos < S.hi && (ns || (pos >= HALO && pos < nx - HALO))) {
+        const uint32_t o = static_cast<uint32_t>(S.t) * NN + static_cast<uint32_t>(ci) * NS + static_cast<uint32_t>(cj);
+        *at32(a.u_out, o) = un;
+        *at32(a.v_out, o) = vn;
+      }
+    }
+  }
+  __syncthreads();
+
+  // halo fill: the velocity from the other side's frame cells in LDS, the new
+  // height from the region (where frame_cell2 formed it from the plan) or,
+  // on corner cells, from the other side's region values at the sources
+  // edge lanes: lane pair k takes owned position p0 + k, its even lane side
+  // A, its odd lane side B
+  const bool eact = tid < 2 * (p1 - p0);
+  const int ex = tid & 1, epa = p0 + (tid >> 1);
+  const int epos = ex ? (flip ? nx - 1 - epa : epa) : epa;
+  const ESide E = ex ? B : A;
+  T edz = T(0);
+  if (eact && en.last) edz = eside_dz(en, E, epos);
+  if (hact) {
+    const ESide Y = hs ? A : B;  // the sources lie on the other side
+    const int y = hs ? 0 : 1;
+    auto src = [&](uint32_t o, int& c, int& pnt) {
+      const uint32_t r = o - static_cast<uint32_t>(Y.t) * NN;
+      const int i = static_cast<int>(r / NS), j = static_cast<int>(r - (r / NS) * NS);
+      c = min(max(eside_cell(Y, i, j), 0), 3 * Y.m - 1);
+      pnt = min(max(eside_point(Y, i, j), 0), (Y.ft.crows + 6) * (Y.ft.ccols + 6) - 1);
+    };
+    int q0, q1, e0, e1;
+    src(hs0, q0, e0);
+    src(hs1, q1, e1);
+    T Rl[kPlanRot];
+#pragma unroll
+    for (int k = 0; k < kPlanRot; ++k) Rl[k] = L.rot[k][tid];
+    T uo, vo;
+    halo_uv_combine(Rl, hc0, hc1, L.ur[y][q0], L.vr[y][q0], L.ur[y][q1], L.vr[y][q1], uo, vo);
+    const T hh = hlayer ? L.s[hs][RP::hn][eside_point(H, hrow, hcol)]
+                        : hc0 * L.s[y][RP::hn][e0] + hc1 * L.s[y][RP::hn][e1];
+    *at32(a.u_out, hdst) = uo;
+    *at32(a.v_out, hdst) = vo;
+    *at32(a.h_out, hdst) = hh;
+    if (hlayer) {
+      L.uh[hs][hlayer - 1][hpos - H.lo] = uo;
+      L.vh[hs][hlayer - 1][hpos - H.lo] = vo;
+    }
+  }
+  __syncthreads();
+
+  // the next stage's edge values of the owned positions: each lane its
+  // side's one-sided value, the pair's sum (IEEE addition commutes, so both
+  // lanes hold the same bits) halved
+  if (eact) {
+    T hv[6];
+    const T vel = eside_vel<T, GSYM>(L, ex, E, N, epos, hv);
+    const T avg = (vel + pair_swap(vel)) / T(2.0);
+    const int o = (E.t * 4 + E.d) * nx + epos;
+    en.evel[o] = avg;
+    if (en.last) {
+      const T f = eside_flux(hv, avg, edz);
+      en.eflux[o] = (f + pair_swap(f)) / T(2.0);
+    }
+  }
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(kFrameEdgeThreads, 3) void k_frame_edge(MarchArgs<T> a, EdgeArgs<T> en) {
+  __shared__ FrameEdgeLds<T, GSYM> L;
+  frame_edge_body<T, GSYM>(a, en, static_cast<int>(blockIdx.x), L);
 }
 
 // Kernel arguments of one RK stage of one context.
@@ -2313,6 +2871,8 @@ struct sw_ctx {
   int gsym = 0;            // a_gct symmetric with equal diagonal everywhere
   void* plan = nullptr;    // halo plan (uint32 dst, s0, s1; T c0, c1, rot[12]), plan_n entries
   int plan_n = 0;
+  int frame_edge_ok = 0;   // the plan's sources fit k_frame_edge (within 3 positions)
+  int frame_edge = 0;      // fused path: k_frame_edge (1) or k_frame + k_halo_uv_edge (0)
   int rows_a = 32;         // rows per wave of k_march_a
   int rows_b = 32;         // rows per wave of k_march_b
   int rows_f = 32;         // rows per wave of k_fused (sw_create sets default_rows_f); 0 selects k_march_a + k_march_b
@@ -2403,9 +2963,9 @@ enum { K_EDGE = 0, K_MARCH_A = 1, K_HALO_H = 2, K_MARCH_B = 3, K_HALO_UV = 4, K_
 // (hipExtLaunchKernelGGL), so the events time the kernel itself with no
 // marker packets between kernels.
 template <typename K, typename... A>
-int launch(const sw_ctx* c, int cls, K kernel, dim3 grid, const A&... args) {
+int launch_n(const sw_ctx* c, int cls, K kernel, dim3 grid, int threads, const A&... args) {
   if (!c->prof) {
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, args...);
+    hipLaunchKernelGGL(kernel, grid, dim3(threads), 0, c->stream, args...);
     SW_HIP(hipGetLastError());
     return SW_OK;
   }
@@ -2414,10 +2974,15 @@ int launch(const sw_ctx* c, int cls, K kernel, dim3 grid, const A&... args) {
     SW_HIP(hipEventCreate(&rec.a));
     SW_HIP(hipEventCreate(&rec.b));
     c->prof->push_back(rec);
-    hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, c->stream, rec.a, rec.b, 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, dim3(threads), 0, c->stream, rec.a, rec.b, 0, args...);
     SW_HIP(hipGetLastError());
   }
   return SW_OK;
+}
+
+template <typename K, typename... A>
+int launch(const sw_ctx* c, int cls, K kernel, dim3 grid, const A&... args) {
+  return launch_n(c, cls, kernel, grid, 256, args...);
 }
 
 template <typename T>
@@ -2474,11 +3039,22 @@ int build_plan(sw_ctx* c) {
   c->grid->plan = c->plan;
   c->plan_n = static_cast<int>(n);
   const HaloPlan<T> p = plan_of<T>(c);
+  int* far = nullptr;
+  SW_HIP(hipMalloc(&far, sizeof(int)));
   PlanBuildArgs<T> a{geo_of(c), route(), planes_of<T>(c), const_cast<uint32_t*>(p.dst),
                      const_cast<uint32_t*>(p.s0), const_cast<uint32_t*>(p.s1), const_cast<T*>(p.c0),
-                     const_cast<T*>(p.c1), const_cast<T*>(p.rot), c->plan_n};
-  hipLaunchKernelGGL(k_plan_build<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
-  SW_HIP(hipGetLastError());
+                     const_cast<T*>(p.c1), const_cast<T*>(p.rot), c->plan_n, far};
+  int hfar = -1;
+  hipError_t e = hipMemsetAsync(far, 0, sizeof(int), c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_plan_build<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&hfar, far, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(far);
+  if (e != hipSuccess) return fail(SW_E_HIP, std::string("halo plan build: ") + hipGetErrorString(e));
+  c->frame_edge_ok = c->frame_edge = hfar == 0 ? 1 : 0;
   return SW_OK;
 }
 
@@ -2615,6 +3191,13 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
     else
       rc = sf ? lf(k_fused<T, false, true>) : lf(k_fused<T, false, false>);
     if (rc) return rc;
+    if (c->frame_edge) {
+      // frame cells, velocity and height halo fill and the next edge pass in
+      // one launch
+      const dim3 ge(12 * ((c->nx + kSeg - 1) / kSeg));
+      return c->gsym ? launch_n(c, K_FRAME, k_frame_edge<T, true>, ge, kFrameEdgeThreads, A.mf, A.en)
+                     : launch_n(c, K_FRAME, k_frame_edge<T, false>, ge, kFrameEdgeThreads, A.mf, A.en);
+    }
     // frame cells and the new height's halo fill in one launch
     rc = c->gsym ? launch(c, K_FRAME, k_frame<T, true>, gr, A.mf, A.hh, nft)
                  : launch(c, K_FRAME, k_frame<T, false>, gr, A.mf, A.hh, nft);
@@ -3573,6 +4156,8 @@ int sw_create_like(sw_ctx** out, const sw_ctx* base) {
   c->locs = base->locs;
   c->plan = base->plan;
   c->plan_n = base->plan_n;
+  c->frame_edge_ok = base->frame_edge_ok;
+  c->frame_edge = base->frame_edge;
   c->dzx_s = base->dzx_s;
   c->dzy_s = base->dzy_s;
   c->dz_uniform = base->dz_uniform;
@@ -3769,6 +4354,22 @@ int sw_set_band_split(sw_ctx* ctx, int rows_short) {
 int sw_get_band_split(const sw_ctx* ctx, int* rows_short) {
   if (!ctx || !rows_short) return fail(SW_E_ARG, "null argument");
   *rows_short = ctx->rows_f2;
+  return SW_OK;
+}
+
+int sw_set_frame_form(sw_ctx* ctx, int cube_edge) {
+  if (!ctx) return fail(SW_E_ARG, "ctx is null");
+  if (cube_edge != 0 && cube_edge != 1) return fail(SW_E_ARG, "cube_edge must be 0 or 1");
+  if (cube_edge && !ctx->frame_edge_ok)
+    return fail(SW_E_ARG, "the grid's halo sources lie farther than 3 positions: cube-edge frame form unavailable");
+  if (cube_edge == ctx->frame_edge) return SW_OK;
+  ctx->frame_edge = cube_edge;
+  return drop_graphs(ctx);
+}
+
+int sw_get_frame_form(const sw_ctx* ctx, int* cube_edge) {
+  if (!ctx || !cube_edge) return fail(SW_E_ARG, "null argument");
+  *cube_edge = ctx->frame_edge;
   return SW_OK;
 }
 

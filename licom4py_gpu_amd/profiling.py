@@ -54,6 +54,8 @@ def _run(args) -> int:
         ctx.set_tuning(*args.rows)
     if args.split is not None:
         ctx.set_band_split(args.split)
+    if args.frame_form is not None:
+        ctx.set_frame_form(bool(args.frame_form))
     t0 = time.perf_counter()
     out = ctx.rk(state[:6], args.substeps, dt, graph=args.graph)
     torch.cuda.synchronize()
@@ -115,6 +117,24 @@ def dispatch_phases(db_path: str, kernel: str, tail: int, prev: int = 0):
             "before": (len(before), mean(before)), "others": (len(head), mean(head))}
 
 
+def timeline(db_path: str, last: int):
+    """The last `last` kernel dispatches in start order, per kernel: their
+    count, mean duration and the mean gap from the end of the dispatch before
+    each to its start (us); and the wall-clock span of the whole window."""
+    db = sqlite3.connect(db_path)
+    rows = db.execute("select name, start, end from kernels order by start").fetchall()[-last:]
+    per = {}
+    for k, (name, start, end) in enumerate(rows):
+        n = _short(name)
+        e = per.setdefault(n, [0, 0.0, 0.0])
+        e[0] += 1
+        e[1] += (end - start) / 1e3
+        if k:
+            e[2] += (start - rows[k - 1][2]) / 1e3
+    span = (rows[-1][2] - rows[0][1]) / 1e3 if rows else 0.0
+    return {n: (c, d / c, g / c) for n, (c, d, g) in per.items()}, span
+
+
 def _summarize(args) -> int:
     lines = []
     if args.command_line:
@@ -150,6 +170,15 @@ def _summarize(args) -> int:
                              f"avg {ph['before'][1]:.3f} us; ")
                 text += f"the others (tuner passes, warm-up, served solves) {ph['others'][0]} avg {ph['others'][1]:.3f} us"
                 lines.append(text)
+        if args.timeline:
+            per, span = timeline(path, args.timeline)
+            lines.append("")
+            lines.append(f"Last {args.timeline} dispatches ({span:.1f} us from first start to last end):")
+            lines.append("")
+            lines.append("| kernel | dispatches | avg us | avg gap before it, us |")
+            lines.append("|---|---|---|---|")
+            for n, (c, d, g) in per.items():
+                lines.append(f"| {n} | {c} | {d:.3f} | {g:.3f} |")
         lines.append("")
     text = "\n".join(lines)
     if args.out:
@@ -206,10 +235,14 @@ def main(argv=None) -> int:
     r.add_argument("--rows", type=int, nargs=3, default=None, help="rows_a rows_b rows_f (0: separate kernels)")
     r.add_argument("--split", type=int, default=None, help="short band height of the fused kernel (0: uniform)")
     r.add_argument("--graph", action="store_true", help="replay recorded substep-pair graphs")
+    r.add_argument("--frame-form", type=int, choices=(0, 1), default=None,
+                   help="1: cube-edge frame launch, 0: frame launch and halo / edge launch")
     s = sub.add_parser("summarize")
     s.add_argument("db", nargs="+")
     s.add_argument("--out", default=None)
     s.add_argument("--command-line", default="")
+    s.add_argument("--timeline", type=int, default=0,
+                   help="also summarize the last N dispatches: duration and gap before each, per kernel")
     s.add_argument("--phases", nargs="*",
                    help="KERNEL:N[:M] - split the KERNEL dispatches into the last N, the M before them and the rest")
     t = sub.add_parser("traffic")

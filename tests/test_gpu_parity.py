@@ -269,6 +269,58 @@ def test_band_split_does_not_change_results(nx, rows_f, split):
         _assert_exact(f"split {rows_f}/{split} rk2 field {idx}", _host(got[k]), want[idx])
 
 
+def _poison_cache(torch, nbytes):
+    """Leave NaN-filled blocks in torch's caching allocator, so outputs a run
+    allocates next start as NaN rather than as an earlier run's values."""
+    x = torch.full((nbytes // 8,), float("nan"), dtype=torch.float64, device="cuda")
+    del x
+
+
+@pytest.mark.parametrize("nx", [8, 15, 16, 22, 48, 140])
+@pytest.mark.parametrize("cube_edge", [True, False])
+def test_frame_forms_match_oracle(nx, cube_edge):
+    """Both forms of the launch after the fused kernel (one cube-edge launch
+    for frame, velocity halo and next edge pass; or the frame launch and a
+    halo / edge launch) give the oracle's bits for a stage and for 3 RK2
+    substeps, eager and in graph replays.  nx = 15, 16 and 22 put the edge of
+    a block's 15 positions at and next to the face end; every halo cell
+    (corners included) is compared."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(nx)
+    ctx = barotr.SWContext(consts, torch.float64)
+    assert ctx.frame_form()  # the default where the grid allows it
+    ctx.set_frame_form(cube_edge)
+    assert ctx.frame_form() == cube_edge
+    core = random_core(nx, seed=9)
+    want = B.step_rk_logic(*core, consts, 20.0, 0.0, True)
+    _poison_cache(torch, 64 * 6 * (nx + 6) ** 2 * 8)
+    got = ctx.stage(*[_dev(x, torch) for x in core], 20.0, 0.0, True)
+    for name, gv, wv in zip(NAMES13, got, want):
+        _assert_exact(f"C{nx} cube_edge={cube_edge} {name}", _host(gv), wv)
+    want = B.barotr_rk2_core(state, consts, 3, dt)
+    for graph in (False, True):
+        _poison_cache(torch, 64 * 6 * (nx + 6) ** 2 * 8)
+        got = ctx.rk([_dev(x, torch) for x in state[:6]], 3, dt, graph=graph)
+        for k, idx in enumerate((0, 2, 3)):
+            _assert_exact(f"C{nx} cube_edge={cube_edge} graph={graph} rk2 field {idx}", _host(got[k]), want[idx])
+
+
+def test_frame_forms_agree_in_fp32():
+    """The FP32 forms evaluate the same fused multiply-adds: both frame forms
+    give the same bits over 4 RK2 substeps at C48."""
+    torch = _torch()
+    from licom4py_gpu_amd import barotr
+    state, consts, dt = seeded_state(48)
+    ctx = barotr.SWContext(consts, torch.float32)
+    runs = []
+    for cube_edge in (True, False):
+        ctx.set_frame_form(cube_edge)
+        runs.append(ctx.rk([_dev(x, torch, torch.float32) for x in state[:6]], 4, dt, graph=True))
+    for a, b in zip(*runs):
+        assert torch.equal(a, b)
+
+
 def test_default_tuning_read_back():
     """sw_create picks the fused kernel's band height from the resolution and
     the device; the context reports what the library holds."""
