@@ -1934,6 +1934,13 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
   struct Pt {
     T u, v, g00, g01, g10, g11, sina, h, hn;
   };
+  // the stencils read the tile's rows across the region and its columns
+  // across the region, not the region's four 3 x 3 corner blocks
+  auto read = [&](int e) {
+    const int rr = rcols == 9 ? e / 9 : rcols == kTile + 6 ? e / (kTile + 6) : e / rcols;
+    const int cc = e - rr * rcols;
+    return (rr >= 3 && rr < 3 + crows) || (cc >= 3 && cc < 3 + ccols);
+  };
   auto fetch = [&](int e, Pt& p) {
     const int rr = rcols == 9 ? e / 9 : rcols == kTile + 6 ? e / (kTile + 6) : e / rcols;
     const int cc = e - rr * rcols;
@@ -1972,11 +1979,12 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
   };
   static_assert(kTilePts <= 2 * kFrameGroup2, "two region points per thread");
   const int e1 = ltid + kFrameGroup2;
+  const bool r0 = ltid < npts && read(ltid), r1 = e1 < npts && read(e1);
   Pt pt0, pt1;
-  fetch(min(ltid, npts - 1), pt0);
-  fetch(min(e1, npts - 1), pt1);
-  if (ltid < npts) put(ltid, pt0);
-  if (e1 < npts) put(e1, pt1);
+  if (r0) fetch(ltid, pt0);
+  if (r1) fetch(e1, pt1);
+  if (r0) put(ltid, pt0);
+  if (r1) put(e1, pt1);
   pre();
   __syncthreads();
   if (q0 >= crows * ccols) return false;  // whole pairs
