@@ -111,6 +111,11 @@ def cpu_baseline(nx: int, substeps: int):
     }
 
 
+def note(text: str) -> None:
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"bench.py: {text}", file=sys.stderr, flush=True)
+
+
 def load_traffic(kernel: str, workload: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
@@ -154,6 +159,7 @@ def main() -> int:
     else:
         tuning = {"rows_a": ctx.tuning[0], "rows_b": ctx.tuning[1], "rows_f": ctx.tuning[2], "rows_f2": ctx.split}
     setup_s = time.perf_counter() - t_setup
+    note(f"set up and tuned in {setup_s:.1f} s: {tuning['rows_f']}/{tuning['rows_f2']} rows")
 
     # served solves first (they run both precisions at many resolutions)
     serving = None
@@ -172,6 +178,7 @@ def main() -> int:
         if not rep["finite"]:
             print("bench.py: non-finite served result", file=sys.stderr)
             return 1
+        note(f"served {args.serve_requests} requests: p99 {serving['p99_s']} s")
 
     # the FP32 form next (its accuracy reference runs FP64 too), so the FP64
     # timed steps and their event-timed graph pass are the last FP64
@@ -181,6 +188,7 @@ def main() -> int:
     if args.dtype == "f64" and fp32_steps > 0 and not args.no_graph:
         fp32 = fp32_leg(consts, state, dt, nbb, fp32_steps, args.warmup, args.accuracy_substeps,
                         args.profile_replays, not args.no_autotune, 6 * nx * nx, f"C{nx}_f32")
+        note(f"FP32 leg: {fp32['value']:.4g} cell-substeps/s")
 
     def step(s):
         return barotr.barotr_rk2_core(s, consts, nbb, dt) if not args.no_graph else \
@@ -202,6 +210,7 @@ def main() -> int:
 
     cells = 6 * nx * nx
     value = cells * nbb * args.steps / el
+    note(f"timed {args.steps} steps: {value:.4g} cell-substeps/s")
     ms_per_step = el / args.steps * 1e3
     path_bytes = PATH_BYTES[args.dtype]
     workload = f"C{nx}_{args.dtype}"
@@ -218,6 +227,7 @@ def main() -> int:
 
     cpu = None
     if args.cpu_substeps > 0:
+        note("CPU baseline sample")
         cpu = cpu_baseline(args.cpu_nx or nx, args.cpu_substeps)
 
     line = {
