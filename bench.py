@@ -81,8 +81,9 @@ def parse():
     p.add_argument("--cpu-substeps", type=int, default=8,
                    help="substeps of the CPU oracle sample (0 disables)")
     p.add_argument("--cpu-nx", type=int, default=None, help="resolution of the CPU sample (default --nx)")
-    p.add_argument("--serve-requests", type=int, default=60,
-                   help="requests of the served trace (Poisson 200/s, C48-C768, 50-5000 steps; 0 disables)")
+    p.add_argument("--serve-requests", type=int, default=200,
+                   help="requests of the served trace (Poisson 200/s, C48-C768, 50-5000 steps, seed 7: the "
+                        "200-request served workload; 0 disables)")
     return p.parse_args()
 
 
