@@ -1877,12 +1877,24 @@ __device__ __forceinline__ T pair_swap(T x) {
   }
 }
 
-// frame_cell with two lanes per cell, for groups of 128 threads: lane 0 of a
+// frame_cell with two lanes per cell, for groups of kEdgeGroup threads and
+// tiles of up to kEdgeTile positions: lane 0 of a
 // pair evaluates the cell's bottom i-face (row ii+1), its j-face at j and the
 // pressure gradient along i, lane 1 the top i-face (row ii), the j-face at
 // j+1 and the gradient along j; lane 0 completes the update.  The same
 // expressions on the same operands as frame_cell.
-constexpr int kFrameGroup2 = 128;
+// k_frame_edge's geometry: kSeg owned positions per block and side, tiles of
+// up to kEdgeTile = kSeg + 6 positions (3 rows), regions of kEdgePts points,
+// kEdgeGroup threads per side (two per cell), kHaloLanes halo-entry lanes
+// per side (3 kSeg strip cells and up to two 9-cell corners).
+constexpr int kSeg = 42;
+constexpr int kEdgeTile = kSeg + 6;
+constexpr int kEdgePts = (kEdgeTile + 6) * 9;
+constexpr int kEdgeGroup = (6 * kEdgeTile + 63) / 64 * 64;
+constexpr int kHaloLanes = (3 * kSeg + 18 + 63) / 64 * 64;
+constexpr int kFrameEdgeThreads = 2 * kEdgeGroup;
+static_assert(kEdgePts <= 2 * kEdgeGroup, "two region points per thread");
+static_assert(2 * kHaloLanes <= kFrameEdgeThreads, "halo entries fit the block");
 
 // Planes of frame_cell2's LDS region: u v g00 g01 [g10 g11] sina h hn (the
 // a_gct planes g10, g11 only when the metric is not symmetric).
@@ -1895,7 +1907,7 @@ struct RegionPlanes {
 // `pre` runs after the region loads are issued and before the barrier.
 template <typename T, bool GSYM, typename Pre>
 __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPlan<T>& plan, const FTile& ft,
-                                            T (*s)[kTilePts], int ltid, T& un, T& vn, int& ci, int& cj, int& cq,
+                                            T (*s)[kEdgePts], int ltid, T& un, T& vn, int& ci, int& cj, int& cq,
                                             Pre&& pre) {
   using RP = RegionPlanes<GSYM>;
   const int nx = a.g.nx, N = a.g.N;
@@ -1929,20 +1941,21 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
     cin[0] = ldg(P.ddx, mo + oc + 1); cin[1] = ldg(P.rdx, mo + oc); cin[2] = ldg(P.rdy, mo + oc); cin[3] = ldg(P.af, fo + oc);
     cin[4] = ldg(a.up, fo + oc); cin[5] = ldg(a.vp, fo + oc);
   }
-  // the region (at most kTilePts = 2 x 128 - 13 points): each thread loads
-  // points ltid and ltid + 128, every load issued before the first LDS write
+  // the region (at most kEdgePts <= 2 kEdgeGroup points): each thread loads
+  // points ltid and ltid + kEdgeGroup, every load issued before the first
+  // LDS write
   struct Pt {
     T u, v, g00, g01, g10, g11, sina, h, hn;
   };
   // the stencils read the tile's rows across the region and its columns
   // across the region, not the region's four 3 x 3 corner blocks
   auto read = [&](int e) {
-    const int rr = rcols == 9 ? e / 9 : rcols == kTile + 6 ? e / (kTile + 6) : e / rcols;
+    const int rr = rcols == 9 ? e / 9 : rcols == kEdgeTile + 6 ? e / (kEdgeTile + 6) : e / rcols;
     const int cc = e - rr * rcols;
     return (rr >= 3 && rr < 3 + crows) || (cc >= 3 && cc < 3 + ccols);
   };
   auto fetch = [&](int e, Pt& p) {
-    const int rr = rcols == 9 ? e / 9 : rcols == kTile + 6 ? e / (kTile + 6) : e / rcols;
+    const int rr = rcols == 9 ? e / 9 : rcols == kEdgeTile + 6 ? e / (kEdgeTile + 6) : e / rcols;
     const int cc = e - rr * rcols;
     const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
     p.u = ldg(a.u, o);
@@ -1977,8 +1990,7 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
     s[RP::h][e] = p.h;
     s[RP::hn][e] = p.hn;
   };
-  static_assert(kTilePts <= 2 * kFrameGroup2, "two region points per thread");
-  const int e1 = ltid + kFrameGroup2;
+  const int e1 = ltid + kEdgeGroup;
   const bool r0 = ltid < npts && read(ltid), r1 = e1 < npts && read(e1);
   Pt pt0, pt1;
   if (r0) fetch(ltid, pt0);
@@ -2129,9 +2141,9 @@ __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a, HaloArgs<T> hh, i
 // Block b takes cube edge e = b / nseg and the kSeg along-edge positions
 // [p0, p1) of its first face edge (A = (t, d)); the second face edge (B,
 // across the joint) owns the mirrored positions.  Each half of the block
-// (128 threads, two per cell: frame_cell2) evaluates the frame cells of one
-// side over its owned positions widened by 3 on either end and keeps their
-// new velocity in LDS.  A halo cell at an owned position takes its two
+// (kEdgeGroup threads, two per cell: frame_cell2) evaluates the frame cells
+// of one side over its owned positions widened by 3 on either end and keeps
+// their new velocity in LDS.  A halo cell at an owned position takes its two
 // interpolation sources from the other side's frame cells, at most 3
 // positions away (checked when the plan is built), so the block forms the
 // vector fill of the owned halo cells (cube.py:137-153) from LDS and writes
@@ -2144,8 +2156,6 @@ __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a, HaloArgs<T> hh, i
 // (cube.py:96-125).  Every value is the expression the separate kernels
 // evaluate, on the same operands.
 // ---------------------------------------------------------------------------
-constexpr int kSeg = kTile - 6;       // owned positions per block (widened: kTile)
-constexpr int kFrameEdgeThreads = 2 * kFrameGroup2;
 
 // One side of a cube-edge block: face edge (t, d), owned positions [lo, hi),
 // evaluated positions [pa, pa + m) and the frame tile holding them.
@@ -2189,12 +2199,12 @@ __device__ __forceinline__ int eside_point(const ESide& S, int i, int j) {
 
 template <typename T, bool GSYM>
 struct FrameEdgeLds {
-  T s[2][RegionPlanes<GSYM>::n][kTilePts];  // the two sides' regions (frame_cell2's planes)
-  T ur[2][3 * kTile];      // new velocity of the evaluated frame cells
-  T vr[2][3 * kTile];
+  T s[2][RegionPlanes<GSYM>::n][kEdgePts];  // the two sides' regions (frame_cell2's planes)
+  T ur[2][3 * kEdgeTile];  // new velocity of the evaluated frame cells
+  T vr[2][3 * kEdgeTile];
   T uh[2][3][kSeg];        // halo values at the owned positions, layers 1..3
   T vh[2][3][kSeg];
-  T rot[kPlanRot][128];    // rotation coefficients of the halo entries of threads 0..127
+  T rot[kPlanRot][2 * kHaloLanes];  // rotation coefficients of the halo entries
 };
 
 // Contravariant normal velocity (edge_body's stencil value) and new height of
@@ -2276,14 +2286,14 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
   const uint32_t NS = static_cast<uint32_t>(N), NN = NS * NS;
 
   // the halo entries of the owned positions and of the corners they carry
-  // (threads 0..127, side tid / 64): their plan data is fetched with the
-  // frame loads, the rotation coefficients staged in LDS
-  const int hs = tid >> 6, hk = tid & 63;
+  // (threads 0 .. 2 kHaloLanes - 1, side tid / kHaloLanes): their plan data
+  // is fetched with the frame loads, the rotation coefficients staged in LDS
+  const int hs = tid >= kHaloLanes ? 1 : 0, hk = tid - hs * kHaloLanes;
   const ESide H = hs ? B : A;
   const int nown = H.hi - H.lo;
   const bool we = H.d == DW || H.d == DE;
   const int c_lo = (we && H.lo == 0) ? 1 : 0, c_hi = (we && H.hi == nx) ? 1 : 0;
-  const bool hact = tid < 128 && hk < 3 * nown + 9 * (c_lo + c_hi);
+  const bool hact = tid < 2 * kHaloLanes && hk < 3 * nown + 9 * (c_lo + c_hi);
   int hq = 0, hlayer = 0, hpos = 0, hrow = 0, hcol = 0;
   if (hact) {
     if (hk < 3 * nown) {
@@ -2320,11 +2330,11 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
 
   // frame cells of both sides
   {
-    const int side = tid >= kFrameGroup2 ? 1 : 0;
+    const int side = tid >= kEdgeGroup ? 1 : 0;
     const ESide S = side ? B : A;
     T un, vn;
     int ci, cj, cq;
-    if (frame_cell2<T, GSYM>(a, plan, S.ft, L.s[side], tid - side * kFrameGroup2, un, vn, ci, cj, cq, stage_rot)) {
+    if (frame_cell2<T, GSYM>(a, plan, S.ft, L.s[side], tid - side * kEdgeGroup, un, vn, ci, cj, cq, stage_rot)) {
       L.ur[side][cq] = un;
       L.vr[side][cq] = vn;
       const bool ns = S.d == DS || S.d == DN;

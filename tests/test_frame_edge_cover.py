@@ -1,7 +1,7 @@
 """The cube-edge frame kernel's block decomposition, modelled on the CPU.
 
 k_frame_edge (csrc/barotr.hip: frame_edge_body, eside) gives block b the
-cube edge b / nseg and 15 along-edge positions of its first face edge; the
+cube edge b / nseg and 42 along-edge positions (kSeg) of its first face edge; the
 face edge across the joint owns the mirrored positions.  Each side evaluates
 its frame cells over the owned positions widened by 3 (its tile), fills the
 velocity and height halo of the owned positions (and, at positions 0 / nx-1
@@ -28,7 +28,7 @@ from licom4py_gpu_amd import _hip, duogrid
 
 DE, DW, DN, DS = 0, 1, 2, 3
 HALO = 3
-SEG = 15  # owned positions per block
+SEG = 42  # owned positions per block (kSeg)
 
 
 def _halo_entry(nx, N, t, row, col):
@@ -123,7 +123,7 @@ def test_frame_edge_blocks_cover_every_cell_once(nx):
              _side(nx, N, tb, db, nx - p1 if flip else p0, nx - p0 if flip else p1))
         for X in S:
             cr0, crows, cc0, ccols = X["tile"]
-            assert crows * ccols <= 63 and (crows + 6) * (ccols + 6) <= 243  # LDS sizes
+            assert crows * ccols <= 3 * (SEG + 6) and (crows + 6) * (ccols + 6) <= (SEG + 12) * 9  # LDS sizes
             ns = X["d"] in (DS, DN)
             for (t, i, j) in X["cells"]:
                 pos = j - 3 if ns else i - 3
@@ -135,7 +135,7 @@ def test_frame_edge_blocks_cover_every_cell_once(nx):
             nown = H["hi"] - H["lo"]
             we = H["d"] in (DW, DE)
             c_lo, c_hi = int(we and H["lo"] == 0), int(we and H["hi"] == nx)
-            assert 3 * nown + 9 * (c_lo + c_hi) <= 64  # one 64-lane group per side
+            assert 3 * nown + 9 * (c_lo + c_hi) <= (3 * SEG + 18 + 63) // 64 * 64  # halo lanes per side
             for k in range(3 * nown + 9 * (c_lo + c_hi)):
                 if k < 3 * nown:
                     layer, pos = k // nown + 1, H["lo"] + k % nown
