@@ -23,8 +23,9 @@
 //                   averaged over the two faces (remap.py:107 ->
 //                   communication.py:485-564) and, for a last stage, the
 //                   averaged mass flux (barotr_pure.py:116-117).
-// With the frame form 0 (sw_set_frame_form), and in batched runs (the *_batch
-// kernels), k_frame_edge is replaced by two launches:
+// Batched runs (the *_batch kernels) use k_frame_edge_batch when every member's
+// context uses the cube-edge form.  With the frame form 0 (sw_set_frame_form),
+// k_frame_edge is replaced by two launches:
 //   k_frame         velocity on the 3-cell frame along the face edges, with
 //                   the new height's duo-grid halo values formed from the halo
 //                   plan, and the halo fill of the new height itself
@@ -2426,7 +2427,7 @@ struct StageArgs {
 // a device table of StageArgs.
 // ---------------------------------------------------------------------------
 enum { BK_EDGE = 0, BK_MARCH_A = 1, BK_HALO = 2, BK_MARCH_B = 3, BK_FUSED = 4, BK_FRAME = 5, BK_HUVE = 6,
-       BK_CLASSES = 7 };
+       BK_FEDGE = 7, BK_CLASSES = 8 };
 
 template <typename T>
 struct BatchLaunch {
@@ -2518,6 +2519,16 @@ __global__ __launch_bounds__(256) void k_halo_uv_edge_batch(BatchLaunch<T> b) {
     halo_vector_body(e.huv, local * 256 + static_cast<int>(threadIdx.x));
   else
     edge_body(e.en, (local - nh) * 256 + static_cast<int>(threadIdx.x));
+}
+
+// Each member's cube-edge frame launch (as k_frame_edge).
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(kFrameEdgeThreads, 3) void k_frame_edge_batch(BatchLaunch<T> b) {
+  __shared__ FrameEdgeLds<T, GSYM> L;
+  const int blk = static_cast<int>(blockIdx.x);
+  const int m = b.blk2m[blk];
+  const StageArgs<T>& e = batch_entry(b, m);
+  frame_edge_body<T, GSYM>(e.mf, e.en, blk - b.first[m], L);
 }
 
 // ---------------------------------------------------------------------------
@@ -3749,6 +3760,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
   std::vector<const void*> ins;
   bool gsym = true;
   bool fused = true;  // the fused stage kernel when every member's context uses it
+  bool fedge = true;  // and the cube-edge frame launch when every member's uses that
   for (int i = 0; i < count; ++i) {
     const sw_member& m = mem[i];
     if (!m.ctx) return fail(SW_E_ARG, "member " + std::to_string(i) + ": ctx is null");
@@ -3769,6 +3781,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     for (int k = 0; k < 6; ++k) ins.push_back(m.in[k]);
     gsym = gsym && m.ctx->gsym;
     fused = fused && m.ctx->rows_f > 0;
+    fedge = fedge && m.ctx->frame_edge;
   }
   {
     std::vector<const sw_ctx*> ctxs;
@@ -3855,7 +3868,8 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
                                     (A.mb.nwaves + 3) / 4, (A.mf.nwaves + 3) / 4,
                                     6 * frame_tiles(c->nx) + static_cast<int>(blocks_for(c->plan_n)),
                                     static_cast<int>(blocks_for(c->plan_n) +
-                                                     blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx)))};
+                                                     blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))),
+                                    12 * ((c->nx + kSeg - 1) / kSeg)};
     for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
   }
   // byte layout of the device copy
@@ -3948,6 +3962,16 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
           hipLaunchKernelGGL((k_fused_batch<T, false, true>), gf, dim3(256), 0, s, L[BK_FUSED]);
         else
           hipLaunchKernelGGL((k_fused_batch<T, false, false>), gf, dim3(256), 0, s, L[BK_FUSED]);
+        if (fedge) {
+          // frame cells, both halo fills and the next edge pass per cube edge
+          const dim3 ge(first[BK_FEDGE][active]);
+          if (gsym)
+            hipLaunchKernelGGL((k_frame_edge_batch<T, true>), ge, dim3(kFrameEdgeThreads), 0, s, L[BK_FEDGE]);
+          else
+            hipLaunchKernelGGL((k_frame_edge_batch<T, false>), ge, dim3(kFrameEdgeThreads), 0, s, L[BK_FEDGE]);
+          SW_HIP(hipGetLastError());
+          continue;
+        }
         // frame cells and the new height's halo fill in one launch
         if (gsym)
           hipLaunchKernelGGL((k_frame_batch<T, true>), gr, dim3(256), 0, s, L[BK_FRAME]);
@@ -3992,7 +4016,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
     auto pair_graph = [&](int active, hipGraphExec_t* out) -> int {
       std::vector<long long> sig = {reinterpret_cast<long long>(b->dev), static_cast<long long>(o_tab),
                                     static_cast<long long>(o_nbb), K, active, rk, gsym ? 1 : 0, fused ? 1 : 0,
-                                    static_cast<long long>(sizeof(T))};
+                                    fedge ? 1 : 0, static_cast<long long>(sizeof(T))};
       for (int k = 0; k < BK_CLASSES; ++k) {
         sig.push_back(static_cast<long long>(o_first[k]));
         sig.push_back(static_cast<long long>(o_blk[k]));
