@@ -183,3 +183,27 @@ def test_nbb_zero_needs_skip_diag():
     out = members[0].ctx.rk(members[0].core6, 0, 10.0, diagnostics=False)
     for got, want in zip(out, (members[0].core6[0], members[0].core6[2], members[0].core6[3])):
         assert torch.equal(got, want)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_frame_forms_match_alone_and_oracle(graph):
+    """Every member on the cube-edge frame launch (k_frame_edge_batch), and a
+    batch where one member's context uses the two-launch form (the whole
+    batch then runs k_frame_batch + k_halo_uv_edge_batch): each member equals
+    its run alone, and the C8 member equals the oracle."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    spec = [(8, 5, 40.0), (16, 4, 30.0), (48, 3, 20.0), (16, 2, 31.0)]
+    for mixed in (False, True):
+        members = _members(torch, torch.float64, spec, seed=11)
+        assert all(m.ctx.frame_form() for m in members)
+        if mixed:
+            members[2].ctx.set_frame_form(False)
+        ensemble.Batch().run(members, graph=graph)
+        _check_alone(torch, members)
+        m = members[0]
+        core = m.tag["core"]
+        st = list(core) + [np.zeros_like(core[0])] * 17
+        want = B.barotr_rk2_core(tuple(st), m.tag["consts"], m.nbb, m.dtb)
+        for i, idx in enumerate((0, 2, 3)):
+            assert np.array_equal(m.out[i].cpu().numpy(), want[idx]), (mixed, idx)
