@@ -229,7 +229,30 @@ struct Planes {
   const int *li, *lj;
   T dzx_s, dzy_s;
   int dz_uniform;
+  // interleaved copies read by the stage kernels as one 2-element load per
+  // point: (g00, g01), (rdx, rdy), (rda, ddx)
+  const T *gp, *rxy, *rdd;
+  // the constant planes of the W and E face-edge strips (the 9 columns next
+  // to each edge) stored row by row, for the cube-edge frame kernel: element
+  // ((((t * 2 + side) * kStripPlanes + k) * N + i) * 9 + c) is plane k at row
+  // i, column c (W) or N - 9 + c (E) of face t
+  const T* strip;
 };
+
+// Planes of the W / E strip copy.
+constexpr int kStripPlanes = 11;
+enum { SP_G00, SP_G01, SP_G10, SP_G11, SP_SINA, SP_CDY, SP_RDA, SP_DDX, SP_RDX, SP_RDY, SP_AF };
+
+
+// Two consecutive elements of an interleaved pair plane: one 16-byte (FP64)
+// or 8-byte (FP32) load per lane.
+template <typename T>
+using Pair = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+template <typename T>
+__device__ __forceinline__ Pair<T> ldg2(const T* base, uint32_t i) {
+  return *reinterpret_cast<const Pair<T>*>(reinterpret_cast<const char*>(base) +
+                                           i * static_cast<uint32_t>(2 * sizeof(T)));
+}
 
 struct Geo {
   int nx, N;
@@ -532,13 +555,16 @@ __device__ __forceinline__ void load_ct2(const MarchArgs<T>& a, uint32_t o, CtRa
   const Planes<T>& P = a.P;
   r.u = ldg(a.u, o);
   r.v = ldg(a.v, o);
-  r.g00 = ldg(P.g00, o);
-  r.g01 = ldg(P.g01, o);
   r.sina = ldg(P.sina, o);
   if (GSYM) {
+    const Pair<T> g = ldg2(P.gp, o);
+    r.g00 = g.x;
+    r.g01 = g.y;
     r.g10 = r.g01;
     r.g11 = r.g00;
   } else {
+    r.g00 = ldg(P.g00, o);
+    r.g01 = ldg(P.g01, o);
     r.g10 = ldg(P.g10, o);
     r.g11 = ldg(P.g11, o);
   }
@@ -582,8 +608,9 @@ __device__ __forceinline__ void a_fetch(const MarchArgs<T>& a, const WaveTile& w
     p.evi = ldg(a.evel, ie);
     p.efi = ldg(a.eflux, ie);
   }
-  p.rda = ldg(P.rda, w.mo + om);
-  p.ddx = ldg(P.ddx, w.mo + om);
+  const Pair<T> rd = ldg2(P.rdd, w.mo + om);
+  p.rda = rd.x;
+  p.ddx = rd.y;
   if constexpr (!SAMEP) p.hp = ldg(a.hp, w.fo + om);
   p.evj = p.efj = T(0);
   if (je && ((w.c0 <= HALO && HALO <= w.c0 + 63) || (w.c0 <= N - HALO && N - HALO <= w.c0 + 63))) {
@@ -1286,7 +1313,8 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       efj = rg.e[es][1][ia - eb];
     }
     // cell inputs of the velocity terms at row ia
-    const T rdx = ldg(P.rdx, w.mo + om), rdy = ldg(P.rdy, w.mo + om), af = ldg(P.af, w.fo + om);
+    const Pair<T> rxy = ldg2(P.rxy, w.mo + om);
+    const T rdx = rxy.x, rdy = rxy.y, af = ldg(P.af, w.fo + om);
     T upl = T(0), vpl = T(0);
     if constexpr (!SAMEP) {
       upl = ldg(a.up, w.fo + om);
@@ -1696,7 +1724,14 @@ __host__ __device__ inline int frame_tiles(int nx) {
 // one side at most kTile long and the other 3.
 struct FTile {
   int t, cr0, crows, cc0, ccols;
+  int strip = -1;  // W (0) / E (1) face-edge tile reading the strip copy of the constant planes, or -1
 };
+
+// Element (row i, strip column c) of strip plane k, face t, side 0 (W) / 1 (E).
+template <typename T>
+__device__ __forceinline__ T strip_ld(const Planes<T>& P, int N, int t, int side, int k, int i, int c) {
+  return ldg(P.strip, static_cast<uint32_t>((((t * 2 + side) * kStripPlanes + k) * N + i) * 9 + c));
+}
 
 // The frame cells of tile ft, evaluated by a group of 256 threads (ltid its
 // thread index in the group): the group loads the region its cells' stencils
@@ -1931,15 +1966,27 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
   const uint32_t fdi = static_cast<uint32_t>(fr) * NS + static_cast<uint32_t>(j);
   const uint32_t fdj = static_cast<uint32_t>(ii) * NS + static_cast<uint32_t>(fc);
   const T fdzi = ldg(P.dzx, P.dz_uniform ? 0 : fo + fdi), fdzj = ldg(P.dzy, P.dz_uniform ? 0 : fo + fdj);
-  const T rpg = kl == 0 ? ldg(P.rdx, mo + oc) : ldg(P.rdy, mo + oc);
-  // cell inputs of the update: lane 0 loads its own, lane 1 the rest
+  // cell inputs of the update: lane 0 loads its own, lane 1 the rest; a W /
+  // E tile takes the constant ones from the strip copy (its 3 columns are
+  // strip columns 3..5: a 128-byte line holds 5 of its rows, not 1)
   const uint32_t on = oc + NS;
+  const int sd = ft.strip, sc = lj;  // strip column of j: the region starts at strip column 0
+  auto cst = [&](const T* plane, uint32_t o, int k, int di, int dc) {
+    return sd >= 0 ? strip_ld(P, N, t, sd, k, ii + di, sc + dc) : ldg(plane, o);
+  };
+  const T rpg = kl == 0 ? cst(P.rdx, mo + oc, SP_RDX, 0, 0) : cst(P.rdy, mo + oc, SP_RDY, 0, 0);
   T cin[6];
   if (kl == 0) {
-    cin[0] = ldg(P.cdy, mo + on); cin[1] = ldg(P.cdy, mo + oc); cin[2] = ldg(P.rda, mo + oc); cin[3] = ldg(P.ddx, mo + oc);
+    cin[0] = cst(P.cdy, mo + on, SP_CDY, 1, 0);
+    cin[1] = cst(P.cdy, mo + oc, SP_CDY, 0, 0);
+    cin[2] = cst(P.rda, mo + oc, SP_RDA, 0, 0);
+    cin[3] = cst(P.ddx, mo + oc, SP_DDX, 0, 0);
     cin[4] = T(0); cin[5] = T(0);
   } else {
-    cin[0] = ldg(P.ddx, mo + oc + 1); cin[1] = ldg(P.rdx, mo + oc); cin[2] = ldg(P.rdy, mo + oc); cin[3] = ldg(P.af, fo + oc);
+    cin[0] = cst(P.ddx, mo + oc + 1, SP_DDX, 0, 1);
+    cin[1] = cst(P.rdx, mo + oc, SP_RDX, 0, 0);
+    cin[2] = cst(P.rdy, mo + oc, SP_RDY, 0, 0);
+    cin[3] = cst(P.af, fo + oc, SP_AF, 0, 0);
     cin[4] = ldg(a.up, fo + oc); cin[5] = ldg(a.vp, fo + oc);
   }
   // the region (at most kEdgePts <= 2 kEdgeGroup points): each thread loads
@@ -1961,13 +2008,23 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
     const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
     p.u = ldg(a.u, o);
     p.v = ldg(a.v, o);
-    p.g00 = ldg(P.g00, o);
-    p.g01 = ldg(P.g01, o);
-    if (!GSYM) {
-      p.g10 = ldg(P.g10, o);
-      p.g11 = ldg(P.g11, o);
+    if (sd >= 0) {
+      p.g00 = strip_ld(P, N, t, sd, SP_G00, ra + rr, cc);
+      p.g01 = strip_ld(P, N, t, sd, SP_G01, ra + rr, cc);
+      if (!GSYM) {
+        p.g10 = strip_ld(P, N, t, sd, SP_G10, ra + rr, cc);
+        p.g11 = strip_ld(P, N, t, sd, SP_G11, ra + rr, cc);
+      }
+      p.sina = strip_ld(P, N, t, sd, SP_SINA, ra + rr, cc);
+    } else {
+      p.g00 = ldg(P.g00, o);
+      p.g01 = ldg(P.g01, o);
+      if (!GSYM) {
+        p.g10 = ldg(P.g10, o);
+        p.g11 = ldg(P.g11, o);
+      }
+      p.sina = ldg(P.sina, o);
     }
-    p.sina = ldg(P.sina, o);
     p.h = ldg(a.h, o);
     const int row = ra + rr, col = ca + cc;
     const bool hrow = row < HALO || row >= N - HALO, hcol = col < HALO || col >= N - HALO;
@@ -2184,6 +2241,7 @@ __device__ __forceinline__ ESide eside(int nx, int N, int t, int d, int lo, int 
     S.ft.crows = S.m;
     S.ft.cc0 = d == DW ? HALO : N - 2 * HALO;
     S.ft.ccols = 3;
+    S.ft.strip = d == DW ? 0 : 1;
   }
   return S;
 }
@@ -2823,6 +2881,7 @@ struct RepackArgs {
   T *o_dzx, *o_dzy, *o_rdx, *o_rdy, *o_af, *o_rda, *o_cdy, *o_ddx, *o_sina;
   T *o_g00, *o_g01, *o_g10, *o_g11, *o_c00, *o_c01, *o_c10, *o_c11, *o_l00, *o_l01, *o_l10, *o_l11;
   T *o_k0, *o_k1;
+  T *o_gp, *o_rxy, *o_rdd;  // pair planes, 2 n elements each
   int *o_li, *o_lj;
 };
 
@@ -2853,11 +2912,38 @@ __global__ __launch_bounds__(256) void k_repack(RepackArgs<T> a) {
   a.o_l11[q] = T(a.l2c[4 * q + 3]);
   a.o_k0[q] = T(a.coef[2 * q + 0]);
   a.o_k1[q] = T(a.coef[2 * q + 1]);
+  a.o_gp[2 * q] = T(a.gct[4 * q + 0]);
+  a.o_gp[2 * q + 1] = T(a.gct[4 * q + 1]);
+  a.o_rxy[2 * q] = T(a.rdx[q]);
+  a.o_rxy[2 * q + 1] = T(a.rdy[q]);
+  a.o_rdd[2 * q] = T(a.rda[q]);
+  a.o_rdd[2 * q + 1] = T(a.ddx[q]);
   a.o_li[q] = a.li[q];
   a.o_lj[q] = a.lj[q];
 }
 
 constexpr int kNumPlanes = 23;  // T planes in the context (see Planes)
+constexpr int kPairPlanes = 3;  // interleaved pair planes after them, 2 n elements each
+
+// Elements of the W / E strip copy (after the pair planes).
+__host__ __device__ inline size_t strip_elems(int N) { return static_cast<size_t>(12) * kStripPlanes * N * 9; }
+
+// The W / E strip copy of the constant planes (Planes::strip).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack_strips(Planes<T> P, int N, T* out) {
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= strip_elems(N)) return;
+  const int c = static_cast<int>(q % 9);
+  const size_t r1 = q / 9;
+  const int i = static_cast<int>(r1 % N);
+  const size_t r2 = r1 / N;
+  const int k = static_cast<int>(r2 % kStripPlanes);
+  const int ts = static_cast<int>(r2 / kStripPlanes);
+  const int t = ts >> 1, side = ts & 1;
+  const T* const src[kStripPlanes] = {P.g00, P.g01, P.g10, P.g11, P.sina, P.cdy, P.rda, P.ddx, P.rdx, P.rdy, P.af};
+  const size_t o = (static_cast<size_t>(t) * N + i) * N + (side ? N - 9 + c : c);
+  out[q] = src[k][o];
+}
 
 inline unsigned blocks_for(size_t n, unsigned per = 256) {
   return static_cast<unsigned>((n + per - 1) / per);
@@ -2889,7 +2975,7 @@ struct sw_ctx {
   hipStream_t stream = nullptr;
   size_t cells = 0;        // 6 * N * N
   std::shared_ptr<GridMem> grid;  // owner of planes, locs and plan below
-  void* planes = nullptr;  // kNumPlanes T planes
+  void* planes = nullptr;  // kNumPlanes T planes, then kPairPlanes pair planes
   int* locs = nullptr;     // loc_i, loc_j planes
   void* scratch = nullptr; // 12 T fields: S1, S2 (stage scratch), X, Y (substep ping-pong)
   void* edges = nullptr;   // evel, eflux, edge-average buffer, store sink
@@ -2974,6 +3060,10 @@ Planes<T> planes_of(const sw_ctx* c) {
                                  &P.sina, &P.g00, &P.g01, &P.g10, &P.g11, &P.c00, &P.c01, &P.c10,
                                  &P.c11, &P.l00, &P.l01, &P.l10, &P.l11, &P.k0, &P.k1};
   for (int k = 0; k < kNumPlanes; ++k) *slots[k] = p + k * n;
+  P.gp = p + kNumPlanes * n;
+  P.rxy = P.gp + 2 * n;
+  P.rdd = P.rxy + 2 * n;
+  P.strip = P.rdd + 2 * n;
   P.li = c->locs;
   P.lj = c->locs + n;
   P.dzx_s = T(c->dzx_s);
@@ -3572,7 +3662,7 @@ template <typename T>
 int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
   const size_t n = c->cells;
   c->grid = std::make_shared<GridMem>();
-  SW_HIP(hipMalloc(&c->planes, kNumPlanes * n * sizeof(T)));
+  SW_HIP(hipMalloc(&c->planes, ((kNumPlanes + 2 * kPairPlanes) * n + strip_elems(c->N)) * sizeof(T)));
   c->grid->planes = c->planes;
   SW_HIP(hipMalloc(&c->locs, 2 * n * sizeof(int)));
   c->grid->locs = c->locs;
@@ -3600,9 +3690,16 @@ int do_create_planes(sw_ctx* c, const void* const consts[SW_N_CONSTS]) {
                           &a.o_sina, &a.o_g00, &a.o_g01, &a.o_g10, &a.o_g11, &a.o_c00, &a.o_c01, &a.o_c10,
                           &a.o_c11, &a.o_l00, &a.o_l01, &a.o_l10, &a.o_l11, &a.o_k0, &a.o_k1};
   for (int k = 0; k < kNumPlanes; ++k) *outs[k] = p + k * n;
+  a.o_gp = p + kNumPlanes * n;
+  a.o_rxy = a.o_gp + 2 * n;
+  a.o_rdd = a.o_rxy + 2 * n;
   a.o_li = c->locs;
   a.o_lj = c->locs + n;
   hipLaunchKernelGGL(k_repack<T>, dim3(blocks_for(n)), dim3(256), 0, c->stream, a);
+  SW_HIP(hipGetLastError());
+  const Planes<T> P = planes_of<T>(c);
+  hipLaunchKernelGGL(k_pack_strips<T>, dim3(blocks_for(strip_elems(c->N))), dim3(256), 0, c->stream, P, c->N,
+                     const_cast<T*>(P.strip));
   SW_HIP(hipGetLastError());
   return build_plan<T>(c);
 }

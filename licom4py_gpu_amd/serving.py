@@ -55,7 +55,9 @@ from . import barotr, duogrid, ensemble, initial
 SCRATCH_FIELDS = 12  # stage scratch and substep ping-pong of a context workspace
 STATE_FIELDS = 6     # two resident (h, u, v) buffers per admitted request
 RESULT_FIELDS = 3    # a kept result: one (h, u, v) buffer
-CONST_PLANES = 26    # context planes of a kind (23) + index tables + halo plan, rounded up
+CONST_PLANES = 29    # field-sized const planes of a kind: 23 planes + 3 interleaved pair planes of 2
+STRIP_PLANES = 11    # planes of the W / E face-edge strip copy (9 columns each)
+PLAN_ROT = 12        # rotation coefficients per halo-plan entry
 
 Init = Union[Tuple[torch.Tensor, torch.Tensor, torch.Tensor], Callable[[], Tuple[torch.Tensor, ...]]]
 
@@ -91,8 +93,18 @@ def request_bytes(nx: int, dtype: torch.dtype) -> int:
 
 
 def kind_bytes(nx: int, dtype: torch.dtype) -> int:
-    """Device bytes of a kind's const planes, held while any of its contexts lives."""
-    return CONST_PLANES * field_bytes(nx, dtype)
+    """Device bytes of a kind's consts, held while any of its contexts lives:
+    the field-sized planes, the W / E strip copy, the two int32 index planes
+    and the halo plan (uint32 destination and sources, two weights and the
+    rotation coefficients per entry, each array padded to 256 bytes)."""
+    n = nx + 6
+    esize = 8 if dtype == torch.float64 else 4
+    strip = 12 * STRIP_PLANES * n * 9 * esize
+    locs = 2 * 6 * n * n * 4
+    entries = 6 * (12 * nx + 36)
+    pad = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    plan = 3 * pad(4 * entries) + 2 * pad(esize * entries) + PLAN_ROT * entries * esize + 256
+    return CONST_PLANES * field_bytes(nx, dtype) + strip + locs + plan
 
 
 class HBMLedger:

@@ -45,7 +45,8 @@ def test_served_results_equal_alone():
     from licom4py_gpu_amd.barotr import SWContext
     reqs = _trace(torch)
     # room for the six resolutions' consts and about four requests at a time
-    budget = 4 * serving.request_bytes(16, torch.float64) + 6 * serving.CONST_PLANES * serving.field_bytes(16, torch.float64)
+    budget = 4 * serving.request_bytes(16, torch.float64) + sum(
+        serving.kind_bytes(nx, dt) for nx in (8, 12, 16) for dt in (torch.float32, torch.float64))
     srv = serving.Server(budget_bytes=budget, chunk=4)
     done = srv.run_trace(reqs)
     assert len(done) == len(reqs)

@@ -95,7 +95,11 @@ def test_field_bytes():
     assert field_bytes(48, torch.float64) == 6 * 54 * 54 * 8
     assert field_bytes(48, torch.float32) == 6 * 54 * 54 * 4
     assert request_bytes(48, torch.float64) == (SCRATCH_FIELDS + STATE_FIELDS) * field_bytes(48, torch.float64)
-    assert kind_bytes(48, torch.float32) == CONST_PLANES * field_bytes(48, torch.float32)
+    # the consts' device allocations: planes, W / E strip copy, index planes, halo plan
+    n, entries = 54, 6 * (12 * 48 + 36)
+    strip, locs = 12 * 11 * n * 9 * 4, 2 * 6 * n * n * 4
+    plan = 3 * ((4 * entries + 255) // 256 * 256) + 2 * ((4 * entries + 255) // 256 * 256) + 12 * entries * 4 + 256
+    assert kind_bytes(48, torch.float32) == CONST_PLANES * field_bytes(48, torch.float32) + strip + locs + plan
 
 
 def test_ledger():
