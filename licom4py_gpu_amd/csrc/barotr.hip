@@ -468,6 +468,17 @@ struct MarchArgs {
   unsigned long long* stamps;
 };
 
+// Lane k <- lane k - 1 across the whole wave (DPP wave_shr:1); lane 0
+// receives 0.
+__device__ __forceinline__ int lane_prev_b32(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ double lane_prev(double x) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+  const unsigned lo = static_cast<unsigned>(lane_prev_b32(static_cast<int>(b)));
+  const unsigned hi = static_cast<unsigned>(lane_prev_b32(static_cast<int>(b >> 32)));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(lane_prev_b32(__float_as_int(x))); }
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1272,7 +1283,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
   fetch(f0, pre);
 
   // face f0 only: its flux and the velocity terms of the face
-  T Fprev, cdy_prev, ucx_prev, vcx_prev, Cx_prev;
+  T Fprev, cdy_prev, ucx_prev, vcx_prev, Cx_prev, ku_prev;
   {
     const APre<T> cur = pre;
     fetch(f0 + STEP, pre);
@@ -1287,12 +1298,17 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     ucx_prev = q.uc;
     vcx_prev = q.vt;
     Cx_prev = q.C;
+    ku_prev = T(0.5) * (q.uc * q.uc + q.vt * q.vt);
     shift(cu);
     shift(cv);
     shift(hh);
   }
 
   int sq = 0;  // FIFO slot of rows ia and ib (read before written)
+  // c6 of the u update's new-height window that the next step reuses: the
+  // window slides one row, so one of its two 6-point values is the other
+  // one of the step before (a window of zeros before the first step)
+  T hx_prev = T(0);
   // step n reads the inputs fetched by the step before (cur) and fetches the
   // next step's (nxt); the loop alternates two buffers, so no copy between
   // them is needed
@@ -1354,9 +1370,10 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     const T cdyb = UP ? cdy_prev : cur.cdy, cdyt = UP ? cur.cdy : cdy_prev;
 
     // height at row ia: divergence (agrid.py:44-53), barotr_pure.py:120
-    const T Gn = __shfl_down(G, 1);
-    const T ddxn = __shfl_down(cur.ddx, 1);
-    const T D = cur.rda * (((Fb * cdyb - Ft * cdyt) + Gn * ddxn) - G * cur.ddx);
+    // lane j+1's G * ddx is the j-face term at j+1 (the same product)
+    const T Gd = G * cur.ddx;
+    const T Gdn = __shfl_down(Gd, 1);
+    const T D = cur.rda * (((Fb * cdyb - Ft * cdyt) + Gdn) - Gd);
     const T hnew = (SAMEP ? hh[SA] : cur.hp) - D * a.dt;  // hh[SA]: h at row ia
     const bool arow = ia >= w.r0 && ia < w.r1;
     T* dh = (own && arow) ? at32(a.h_out, w.fo + static_cast<uint32_t>(ia) * NS + j) : sink;
@@ -1376,18 +1393,20 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     // velocity terms at row ia (barotr_pure.py:133-150, agrid.py:32-41):
     // vorticity with vv = vb_cx, uu = ub_cy, kinetic-energy gradient,
     // dissipation and Coriolis
-    const T ucy_n = __shfl_down(qj.vt, 1);
-    const T vcy_n = __shfl_down(qj.uc, 1);
-    const T Cy_n = __shfl_down(qj.C, 1);
     const T ucy = qj.vt, vcy = qj.uc, Cy = qj.C;
+    // lane j+1's products at the j-face j+1: ucy * ddx and the kinetic energy
+    const T Uy = ucy * cur.ddx;
+    const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+    const T Uyn = __shfl_down(Uy, 1);
+    const T kv1 = __shfl_down(kv0, 1);
+    const T Cy_n = __shfl_down(Cy, 1);
     const T ucxb = UP ? ucx_prev : qi.uc, ucxt = UP ? qi.uc : ucx_prev;
     const T vcxb = UP ? vcx_prev : qi.vt, vcxt = UP ? qi.vt : vcx_prev;
     const T Cxb = UP ? Cx_prev : qi.C, Cxt = UP ? qi.C : Cx_prev;
-    const T zeta = cur.rda * (((ucy * cur.ddx + vcxb * cdyb) - ucy_n * ddxn) - vcxt * cdyt);
-    const T ku1 = T(0.5) * (ucxb * ucxb + vcxb * vcxb);
-    const T ku0 = T(0.5) * (ucxt * ucxt + vcxt * vcxt);
-    const T kv1 = T(0.5) * (vcy_n * vcy_n + ucy_n * ucy_n);
-    const T kv0 = T(0.5) * (vcy * vcy + ucy * ucy);
+    const T zeta = cur.rda * (((Uy + vcxb * cdyb) - Uyn) - vcxt * cdyt);
+    // kinetic energy of the i-faces: this step's face and the previous one's
+    const T kuf = T(0.5) * (qi.uc * qi.uc + qi.vt * qi.vt);
+    const T ku1 = UP ? ku_prev : kuf, ku0 = UP ? kuf : ku_prev;
     const T uct_c = cu[SA];  // contravariant u at row ia
     const T advx = zeta * vct_c - (ku1 - ku0) * rdx;
     const T advy = -zeta * uct_c - (kv1 - kv0) * rdy;
@@ -1401,8 +1420,9 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       T sn[7];
 #pragma unroll
       for (int m = 0; m < 7; ++m) sn[m] = rg.b[min(lane + m, kRowBuf - 1)];
-      const T hy0 = c6(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]);
+      // lane j-1's hy1 is this lane's hy0 (the same six values)
       const T hy1 = c6(sn[1], sn[2], sn[3], sn[4], sn[5], sn[6]);
+      const T hy0 = lane_prev(hy1);
       const T pgfv = T(-9.80) * (rdy * (hy1 - hy0));
       const bool vrow = ia >= b0 && ia < b1;
       T* dv = (deep && vrow) ? at32(a.v_out, w.fo + static_cast<uint32_t>(ia) * NS + j) : sink;
@@ -1415,8 +1435,9 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
       T tq[kFTerms];
 #pragma unroll
       for (int k = 0; k < kFTerms; ++k) tq[k] = tq0[k];
-      const T hx0 = c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]);
-      const T hx1 = c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
+      const T hx0 = UP ? c6(wn7[0], wn7[1], wn7[2], wn7[3], wn7[4], wn7[5]) : hx_prev;
+      const T hx1 = UP ? hx_prev : c6(wn7[1], wn7[2], wn7[3], wn7[4], wn7[5], wn7[6]);
+      hx_prev = UP ? hx0 : hx1;
       const T pgfu = T(-9.80) * (tq[4] * (hx1 - hx0));
       const bool urow = ib >= b0 && ib < b1;
       const uint32_t oc = w.fo + static_cast<uint32_t>(min(max(ib, 0), N - 1)) * NS + j;
@@ -1438,6 +1459,7 @@ __device__ __forceinline__ void fused_march(const MarchArgs<T>& a, const WaveTil
     ucx_prev = qi.uc;
     vcx_prev = qi.vt;
     Cx_prev = qi.C;
+    ku_prev = kuf;
     sq = sq == kFifo - 1 ? 0 : sq + 1;
     shift(cu);
     shift(cv);
