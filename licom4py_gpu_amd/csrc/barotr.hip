@@ -186,6 +186,29 @@ __device__ __forceinline__ T contra(T ga, T gb, T u, T v, T sina) {
   return (ga * u + gb * v) * sina;
 }
 
+// FP64 quotient with the operand scaling of the general sequence left out
+// where it is the identity, so the result is the same correctly rounded bits
+// with fewer instructions.  a / b: reciprocal estimate, two Newton steps,
+// quotient and one residual correction, then the special-value fixup.  The compiler's sequence scales
+// both operands (v_div_scale) only when the exponents are far apart, b is
+// denormal or a / b or 1 / b would be: never for a = 4.9 and b in [10, 2^1021]
+// or b = +inf / NaN, the divisors of k49_over.
+__device__ __forceinline__ double div_f64(double a, double b) {
+  const double y0 = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, y0, 1.0);
+  double y = __builtin_fma(y0, e, y0);
+  e = __builtin_fma(-b, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  const double q = a * y;
+  const double r = __builtin_fma(-b, q, a);
+  return __builtin_amdgcn_div_fixup(__builtin_fma(r, y, q), b, a);
+}
+// c >= 10 (or NaN) and c <= sqrt(9.8 DBL_MAX) < 2^1021, or +inf
+template <>
+__device__ __forceinline__ double k49_over<double>(double c) {
+  return div_f64(4.9, c);
+}
+
 // The FP32 form is measured against the FP64 path (relative error), not bit
 // for bit, so it evaluates the same sums as fused multiply-adds in the same
 // association and takes the celerity's square root and reciprocal from the
@@ -1959,7 +1982,10 @@ static_assert(2 * kHaloLanes <= kFrameEdgeThreads, "halo entries fit the block")
 template <bool GSYM>
 struct RegionPlanes {
   static constexpr int u = 0, v = 1, g00 = 2, g01 = 3, g10 = 4, g11 = 5;
-  static constexpr int sina = GSYM ? 4 : 6, h = GSYM ? 5 : 7, hn = GSYM ? 6 : 8, n = GSYM ? 7 : 9;
+  static constexpr int sina = GSYM ? 4 : 6, h = GSYM ? 5 : 7, hn = GSYM ? 6 : 8;
+  // contravariant u and v of the point (remap.py:35-39), formed once when the
+  // point is stored: every face stencil through the point reads them
+  static constexpr int cu = GSYM ? 7 : 9, cv = GSYM ? 8 : 10, n = GSYM ? 9 : 11;
 };
 
 // `pre` runs after the region loads are issued and before the barrier.
@@ -2069,6 +2095,8 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
     s[RP::sina][e] = p.sina;
     s[RP::h][e] = p.h;
     s[RP::hn][e] = p.hn;
+    s[RP::cu][e] = contra(p.g00, p.g01, p.u, p.v, p.sina);
+    s[RP::cv][e] = GSYM ? contra(p.g01, p.g00, p.u, p.v, p.sina) : contra(p.g10, p.g11, p.u, p.v, p.sina);
   };
   const int e1 = ltid + kEdgeGroup;
   const bool r0 = ltid < npts && read(ltid), r1 = e1 < npts && read(e1);
@@ -2090,11 +2118,8 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       const int e = e0 + m * es;
-      const T u = s[RP::u][e], v = s[RP::v][e], sina = s[RP::sina][e];
-      const T g00 = s[RP::g00][e], g01 = s[RP::g01][e];
-      const T g10 = GSYM ? g01 : s[RP::g10][e], g11 = GSYM ? g00 : s[RP::g11][e];
-      wf[m] = iface ? contra(g00, g01, u, v, sina) : contra(g10, g11, u, v, sina);
-      tf[m] = iface ? v : u;
+      wf[m] = s[iface ? RP::cu : RP::cv][e];
+      tf[m] = s[iface ? RP::v : RP::u][e];
       hf[m] = s[RP::h][e];
     }
     const bool fedge = iface ? (r == HALO) || (r == N - HALO) : (c == HALO) || (c == N - HALO);
@@ -2139,11 +2164,8 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
   const T ucx = uci, vcx = vti, Cx = Ci, pgfu = pgf;
   const T vcy = ucj, ucy = vtj, Cy = Cj;
   const int ec = li * rcols + lj;
-  const T uc0 = s[RP::u][ec], vc0 = s[RP::v][ec], sc0 = s[RP::sina][ec];
-  const T g00c = s[RP::g00][ec], g01c = s[RP::g01][ec];
-  const T g10c = GSYM ? g01c : s[RP::g10][ec], g11c = GSYM ? g00c : s[RP::g11][ec];
-  const T vct_c = contra(g10c, g11c, uc0, vc0, sc0);
-  const T uct_c = contra(g00c, g01c, uc0, vc0, sc0);
+  const T vct_c = s[RP::cv][ec];
+  const T uct_c = s[RP::cu][ec];
   const T cdy_i = cin[0], cdy_prev = cin[1], rda = cin[2], ddx = cin[3];
   const T ddx_n = cv[0], rdx = cv[1], rdy = cv[2], af = cv[3];
   const T up = cv[4], vp = cv[5];
