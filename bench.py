@@ -13,8 +13,10 @@ The JSON line also carries
                  cell in FP64 (296 B per cell-substep over the two stages of
                  a substep), over its average launch time as it runs in the
                  timed steps: the substep-pair graph the timed steps replay,
-                 recorded with an event node around every kernel node and
-                 replayed after the timed region; plus the kernel's own byte
+                 recorded with the kernel's wave stamps on (lane 0 of every
+                 wave writes the device wall clock at entry and exit; a launch
+                 lasts from its first entry to its last exit) and replayed
+                 after the timed region; plus the kernel's own byte
                  count (face-shared planes counted once for six faces) and
                  its L2-to-fabric bytes per launch from the committed
                  rocprofv3 counter summary,
