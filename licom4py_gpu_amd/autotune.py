@@ -132,7 +132,9 @@ def _tune_fused_graph(ctx, core6, dt, ra, rb, cands, reps: int = 2):
     best = min((t, int(k.split("/")[0]), 0) for k, t in table.items())
     rf = best[1]
     shares = SPLIT_SHARES[str(ctx.dtype).replace("torch.", "")]
-    for span in sorted({2 * rf - 4, 2 * rf, 2 * rf + 4}):
+    # spans in steps of 2 rows: which span gives as many short-band blocks as
+    # long-band ones (one of each per CU) depends on the resolution
+    for span in sorted({2 * rf - 4, 2 * rf - 2, 2 * rf, 2 * rf + 2, 2 * rf + 4}):
         for share in shares:
             lng = int(round(span * share))
             sh = span - lng
