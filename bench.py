@@ -29,8 +29,10 @@ The JSON line also carries
                  of the same workload,
   serving        p50/p99 latency and throughput of served solves (a seeded
                  Poisson trace of mixed resolutions, step counts and
-                 precisions under continuous batching) beside the plain FP64
-                 path serving the same trace one request at a time.
+                 precisions under continuous batching, least remaining work
+                 first) beside the same trace with every admitted request
+                 advancing each tick, and the plain FP64 path serving it one
+                 request at a time.
 Only one GPU is used: --gpus above 1 exits with a message.
 """
 
@@ -168,17 +170,24 @@ def main() -> int:
     serving = None
     if args.serve_requests > 0:
         from licom4py_gpu_amd import serving as srv
-        rep = srv.serve_report(args.serve_requests, 200.0, [48, 96, 192, 384, 768], (50, 5000), 200.0, 100,
-                               7, dev, baselines=("fp64",))
+        rep = srv.serve_report(args.serve_requests, 200.0, [48, 96, 192, 384, 768], (50, 5000), 200.0, 400,
+                               7, dev, baselines=("fp64",), policy="srpt", width=2e7)
+        # the same trace with every admitted request advancing 100 substeps per tick
+        ps = srv.serve_report(args.serve_requests, 200.0, [48, 96, 192, 384, 768], (50, 5000), 200.0, 100,
+                              7, dev, baselines=(), policy="all")
         base = rep["one_at_a_time_fp64"]
         serving = {"workload": rep["workload"],
+                   "policy": "least remaining work first, ticks of 2e7 cells and up to 400 substeps",
                    "p50_s": round(rep["latency"]["p50_s"], 4), "p99_s": round(rep["latency"]["p99_s"], 4),
                    "throughput_cell_substeps_per_s": rep["throughput_cell_substeps_per_s"],
                    "mean_batch": round(rep["mean_batch"], 2), "peak_hbm_gb": round(rep["peak_hbm_gb"], 3),
+                   "every_request_each_tick": {"p50_s": round(ps["latency"]["p50_s"], 4),
+                                               "p99_s": round(ps["latency"]["p99_s"], 4),
+                                               "throughput_cell_substeps_per_s": ps["throughput_cell_substeps_per_s"]},
                    "plain_fp64_one_at_a_time": {"p50_s": round(base["p50_s"], 4), "p99_s": round(base["p99_s"], 4),
                                                 "throughput_cell_substeps_per_s":
                                                     base["throughput_cell_substeps_per_s"]}}
-        if not rep["finite"]:
+        if not (rep["finite"] and ps["finite"]):
             print("bench.py: non-finite served result", file=sys.stderr)
             return 1
         note(f"served {args.serve_requests} requests: p99 {serving['p99_s']} s")

@@ -39,7 +39,8 @@ def _trace(torch):
     return reqs
 
 
-def test_served_results_equal_alone():
+@pytest.mark.parametrize("policy", ["srpt", "all"])
+def test_served_results_equal_alone(policy):
     torch = _torch()
     from licom4py_gpu_amd import serving
     from licom4py_gpu_amd.barotr import SWContext
@@ -47,7 +48,7 @@ def test_served_results_equal_alone():
     # room for the six resolutions' consts and about four requests at a time
     budget = 4 * serving.request_bytes(16, torch.float64) + sum(
         serving.kind_bytes(nx, dt) for nx in (8, 12, 16) for dt in (torch.float32, torch.float64))
-    srv = serving.Server(budget_bytes=budget, chunk=4)
+    srv = serving.Server(budget_bytes=budget, chunk=4, policy=policy, width=6 * 16 * 16 * 2)
     done = srv.run_trace(reqs)
     assert len(done) == len(reqs)
     assert srv.peak_used <= budget

@@ -113,7 +113,7 @@ def test_ledger():
 
 def test_results_are_the_substeps_and_bytes_return():
     b = StubBackend()
-    srv = serving.Server(1e12, chunk=4, backend=b)
+    srv = serving.Server(1e12, chunk=4, backend=b, policy="all")
     reqs = [_req(0, 8, 10), _req(1, 8, 3), _req(2, 12, 0), _req(3, 12, 7, torch.float32)]
     done = srv.run_trace(reqs)
     assert sorted(r.rid for r in done) == [0, 1, 2, 3] and not srv.failed
@@ -135,7 +135,7 @@ def test_results_are_the_substeps_and_bytes_return():
 
 def test_idle_workspace_is_reused():
     b = StubBackend()
-    srv = serving.Server(1e12, chunk=8, backend=b)
+    srv = serving.Server(1e12, chunk=8, backend=b, policy="all")
     srv.run_trace([_req(0, 8, 2), _req(1, 8, 2, arrival=0.0)])
     srv.run_trace([_req(2, 8, 2)])
     # two contexts of C8 (the second cloned), the third request reuses one
@@ -150,7 +150,7 @@ def test_head_request_evicts_unused_kinds_instead_of_stalling():
     big = (48, torch.float64)
     fb = field_bytes(*big)
     budget = kind_bytes(*big) + request_bytes(*big)  # exactly one C48 FP64 kind and request
-    srv = serving.Server(budget, chunk=4, backend=b)
+    srv = serving.Server(budget, chunk=4, backend=b, policy="all")
     srv.run_trace([_req(0, 16, 2, keep=False), _req(1, 8, 2, torch.float32, keep=False)])
     assert srv.used > 0 and len(srv.kinds) == 2     # two small kinds resident and idle
     assert not srv.ledger.fits(kind_bytes(*big) + request_bytes(*big))
@@ -168,7 +168,7 @@ def test_request_that_can_never_fit_fails_instead_of_looping():
     k = (16, torch.float64)
     fb = field_bytes(*k)
     budget = kind_bytes(*k) + request_bytes(*k) + 2 * RESULT_FIELDS * fb
-    srv = serving.Server(budget, chunk=4, backend=b)
+    srv = serving.Server(budget, chunk=4, backend=b, policy="all")
     first = srv.run_trace([_req(0, 16, 1), _req(1, 16, 1, arrival=0.0)])
     assert len(first) == 2  # two kept results
     # the kind and its workspaces can be freed, but the results stay: a C16
@@ -189,7 +189,7 @@ def test_ticks_run_ahead_of_retirement():
     returned, only when the event of its last tick has completed, with that
     event's time as its completion time."""
     b = StubBackend(hold=True)
-    srv = serving.Server(1e12, chunk=2, backend=b, depth=2)
+    srv = serving.Server(1e12, chunk=2, backend=b, depth=2, policy="all")
     srv.submit(_req(0, 8, 2))
     srv.submit(_req(1, 8, 6))
     clock = iter(np.arange(0.0, 100.0, 0.5))
@@ -232,9 +232,54 @@ def test_surplus_idle_workspaces_of_the_same_kind_are_freed():
     b = StubBackend()
     k = (16, torch.float64)
     fb = field_bytes(*k)
-    srv = serving.Server(1e12, chunk=4, backend=b)
+    srv = serving.Server(1e12, chunk=4, backend=b, policy="all")
     srv.run_trace([_req(i, 16, 1, keep=False) for i in range(3)])  # three contexts, all idle now
     assert len(srv.kinds[k].idle) == 3
     srv.ledger.budget = srv.used + STATE_FIELDS * fb - 1.0  # one byte short of admitting into an idle one
     srv.run_trace([_req(9, 16, 1, keep=False)])
     assert srv.done[-1].rid == 9 and not srv.failed and srv.evictions == 1
+
+
+def _active(srv, rid, nx, remaining, arrival=0.0, dtype=torch.float64):
+    a = serving._Active(req=serving.Request(rid, nx, remaining, dtype, 1.0, (None, None, None), arrival=arrival),
+                        ctx=None, bufs=[None, None], remaining=remaining)
+    srv.active.append(a)
+    return a
+
+
+def test_srpt_plan_least_remaining_work_first_until_the_width():
+    """Requests join the tick in order of cells x remaining substeps until it
+    spans `width` cells; each advances min(chunk, remaining) substeps."""
+    srv = serving.Server(1e12, chunk=10, backend=StubBackend(), width=6 * 16 * 16 + 6 * 8 * 8)
+    _active(srv, 0, 48, 1000)              # 6*48^2 cells x 1000: the most work
+    _active(srv, 1, 8, 5, arrival=1.0)     # 6*8^2 cells x 5: the least
+    _active(srv, 2, 16, 500, arrival=2.0)  # 6*16^2 cells x 500
+    _active(srv, 3, 8, 700, arrival=3.0)   # 6*8^2 cells x 700
+    plan = srv.plan()
+    assert [(a.req.rid, n) for a, n in plan] == [(1, 5), (3, 10), (2, 10)]  # width reached: request 0 waits
+    srv.width = 1.0
+    assert [(a.req.rid, n) for a, n in srv.plan()] == [(1, 5)]  # at least one request per tick
+
+
+def test_all_policy_takes_every_request_and_policy_is_checked():
+    srv = serving.Server(1e12, chunk=5, backend=StubBackend(), policy="all")
+    for k in range(4):
+        _active(srv, k, 16, 100 if k else 3, arrival=float(k))
+    assert [(a.req.rid, n) for a, n in srv.plan()] == [(0, 3), (1, 5), (2, 5), (3, 5)]
+    with pytest.raises(ValueError, match="policy"):
+        serving.Server(1e12, backend=StubBackend(), policy="fifo")
+
+
+def test_srpt_trace_results_and_small_requests_first():
+    """Under "srpt" every result is still its own substeps, and of requests
+    that arrive together the ones with less work finish first."""
+    b = StubBackend()
+    srv = serving.Server(1e12, chunk=2, backend=b, width=6 * 12 * 12)
+    reqs = [_req(0, 12, 9), _req(1, 8, 3), _req(2, 12, 2), _req(3, 8, 8, torch.float32)]
+    done = srv.run_trace(reqs)
+    assert sorted(r.rid for r in done) == [0, 1, 2, 3] and not srv.failed
+    for r in done:
+        for k in range(3):
+            assert torch.all(r.result[k][:, 3:-3, 3:-3] == k + r.steps)
+    order = [r.rid for r in sorted(done, key=lambda r: (r.t_done, r.rid))]
+    assert order.index(1) < order.index(0) and order.index(2) < order.index(0)
