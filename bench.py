@@ -134,7 +134,7 @@ def main() -> int:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) "
+        print(f"bench.py: the integrator runs on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) "
               "is not supported", file=sys.stderr)
         return 2
 

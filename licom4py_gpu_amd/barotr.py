@@ -333,7 +333,7 @@ def context_builds() -> int:
 def _check_mesh(px: int, py: int) -> None:
     if px != 1 or py != 1:
         raise ValueError(
-            f"px={px}, py={py}: this build keeps all six faces on one GPU; "
+            f"px={px}, py={py}: this library keeps all six faces on one GPU; "
             "only px = py = 1 is supported")
 
 

@@ -123,7 +123,7 @@ def test_stage_matches_oracle(nx, last, beta):
 
 
 def test_rk2_c48_seeded_200_substeps():
-    """C48, seeded balanced field, Courant 0.5, 200 substeps
+    """The C48 parity workload: seeded balanced field, Courant 0.5, 200 substeps
     as 4 calls of nbb = 50, all 23 state arrays compared after every call."""
     torch = _torch()
     from licom4py_gpu_amd import barotr
