@@ -77,7 +77,7 @@ def parse():
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default band height instead of timing the candidates")
     p.add_argument("--profile-replays", type=int, default=4,
-                   help="replays of the event-timed substep-pair graph (0 disables)")
+                   help="measured replays of the wave-stamped substep-pair graph (0 disables)")
     p.add_argument("--fp32-steps", type=int, default=None,
                    help="timed steps of the FP32 leg (default --steps; 0 disables)")
     p.add_argument("--accuracy-substeps", type=int, default=10000,
@@ -193,7 +193,7 @@ def main() -> int:
         note(f"served {args.serve_requests} requests: p99 {serving['p99_s']} s")
 
     # the FP32 form next (its accuracy reference runs FP64 too), so the FP64
-    # timed steps and their event-timed graph pass are the last FP64
+    # timed steps and their wave-stamped graph replays are the last FP64
     # stage-kernel launches of the run
     fp32 = None
     fp32_steps = args.steps if args.fp32_steps is None else args.fp32_steps

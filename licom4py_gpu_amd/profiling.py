@@ -99,8 +99,8 @@ def counter_stats(db_path: str):
 
 def dispatch_phases(db_path: str, kernel: str, tail: int, prev: int = 0):
     """Average duration (us) of the kernel's dispatches: all of them, the last
-    `tail` (bench.py's event-timed graph pass runs last), the `prev` before
-    those (bench.py's timed steps) and the others."""
+    `tail` (bench.py's wave-stamped graph replays run last), the `prev`
+    before those (bench.py's timed steps) and the others."""
     db = sqlite3.connect(db_path)
     rows = db.execute("select duration from kernels where name like ? order by start",
                       (f"%{kernel}%",)).fetchall()
@@ -164,7 +164,7 @@ def _summarize(args) -> int:
             ph = dispatch_phases(path, kname, tail, prev)
             if ph:
                 text = (f"Dispatches of {kname}: all {ph['all'][0]} avg {ph['all'][1]:.3f} us; "
-                        f"last {ph['last'][0]} (the event-timed graph pass) avg {ph['last'][1]:.3f} us; ")
+                        f"last {ph['last'][0]} (the wave-stamped graph replays) avg {ph['last'][1]:.3f} us; ")
                 if prev:
                     text += (f"the {ph['before'][0]} before them (graph replays of the timed steps) "
                              f"avg {ph['before'][1]:.3f} us; ")

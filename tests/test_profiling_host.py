@@ -22,7 +22,7 @@ def _trace_db(path, durations_ns):
 
 def test_dispatch_phases_split_last_before_rest(tmp_path):
     path = str(tmp_path / "t.db")
-    # 3 early dispatches (1 us), 4 "timed" (2 us), 2 "event-timed" (5 us), other kernels interleaved
+    # 3 early dispatches (1 us), 4 "timed" (2 us), 2 "graph-replayed" (5 us), other kernels interleaved
     seq = [("k_fused<double>", 1000)] * 3 + [("k_frame<double>", 9000)] + \
           [("k_fused<double>", 2000)] * 4 + [("k_fused<double>", 5000)] * 2
     _trace_db(path, seq)
