@@ -175,6 +175,28 @@ template <typename T>
 __device__ __forceinline__ T celerity(T centred, T dz) {
   return max10<T>(sqrt(T(9.8) * (centred + dz)));
 }
+// FP64 square root as the general sequence evaluates it (reciprocal square
+// root estimate, two Goldschmidt steps, two Newton corrections, +-0 / +inf
+// passed through) without its input scaling, which is the identity for
+// x >= 2^-767: the same bits there.  Below, the unscaled steps give a finite
+// value under 10 for x in (0, 2^-767) and NaN for x < 0, the same results
+// after max10 as the general sequence's.
+__device__ __forceinline__ double sqrt_f64(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return __builtin_amdgcn_class(x, 0x260) ? x : g;  // 0x260: +-0, +inf
+}
+template <>
+__device__ __forceinline__ double celerity<double>(double centred, double dz) {
+  return max10<double>(sqrt_f64(9.8 * (centred + dz)));
+}
 // 4.9 / c of the LMARS correction (barotr_pure.py:33).
 template <typename T>
 __device__ __forceinline__ T k49_over(T c) {
