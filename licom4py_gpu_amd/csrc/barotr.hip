@@ -1980,24 +1980,39 @@ __device__ __forceinline__ T pair_swap(T x) {
   }
 }
 
-// frame_cell with two lanes per cell, for groups of kEdgeGroup threads and
-// tiles of up to kEdgeTile positions: lane 0 of a
+// frame_cell with two lanes per cell, for groups of G::group threads and
+// tiles of up to G::tile positions: lane 0 of a
 // pair evaluates the cell's bottom i-face (row ii+1), its j-face at j and the
 // pressure gradient along i, lane 1 the top i-face (row ii), the j-face at
 // j+1 and the gradient along j; lane 0 completes the update.  The same
 // expressions on the same operands as frame_cell.
-// k_frame_edge's geometry: kSeg owned positions per block and side, tiles of
-// up to kEdgeTile = kSeg + 6 positions (3 rows), regions of kEdgePts points,
-// kEdgeGroup threads per side (two per cell), kHaloLanes halo-entry lanes
-// per side (3 kSeg strip cells and up to two 9-cell corners).
+// k_frame_edge's geometry for SEG owned positions per block and side: tiles
+// of up to tile = SEG + 6 positions (3 rows), regions of pts points, group
+// threads per side (two per cell), halo_lanes halo-entry lanes per side (3 SEG
+// strip cells and up to two 9-cell corners), threads per block.
+template <int SEG>
+struct EdgeGeo {
+  static constexpr int seg = SEG;
+  static constexpr int tile = SEG + 6;
+  static constexpr int pts = (tile + 6) * 9;
+  static constexpr int group = (6 * tile + 63) / 64 * 64;
+  static constexpr int halo_lanes = (3 * SEG + 18 + 63) / 64 * 64;
+  static constexpr int threads = 2 * group;
+  static_assert(pts <= 2 * group, "two region points per thread");
+  static_assert(2 * halo_lanes <= threads, "halo entries fit the block");
+  static_assert(2 * SEG <= threads, "one lane pair per owned position in the edge pass");
+};
+// Segment lengths: single-context launches use 42 positions (228 blocks at
+// C768, one round on the device, where the block's latency sets the time).
+// Batched launches run many rounds, where blocks resident per CU set the
+// throughput: FP32 blocks of 28 positions (40 KB LDS, 512 threads) fit three
+// per CU where those of 42 fit two; FP64 ones of 42 (115 KB) stay one per CU,
+// and shorter FP64 segments measured slower (the margins' loads).
 constexpr int kSeg = 42;
-constexpr int kEdgeTile = kSeg + 6;
-constexpr int kEdgePts = (kEdgeTile + 6) * 9;
-constexpr int kEdgeGroup = (6 * kEdgeTile + 63) / 64 * 64;
-constexpr int kHaloLanes = (3 * kSeg + 18 + 63) / 64 * 64;
-constexpr int kFrameEdgeThreads = 2 * kEdgeGroup;
-static_assert(kEdgePts <= 2 * kEdgeGroup, "two region points per thread");
-static_assert(2 * kHaloLanes <= kFrameEdgeThreads, "halo entries fit the block");
+template <typename T>
+constexpr int batch_seg() {
+  return sizeof(T) == 8 ? 42 : 28;
+}
 
 // Planes of frame_cell2's LDS region: u v g00 g01 [g10 g11] sina h hn (the
 // a_gct planes g10, g11 only when the metric is not symmetric).
@@ -2011,9 +2026,9 @@ struct RegionPlanes {
 };
 
 // `pre` runs after the region loads are issued and before the barrier.
-template <typename T, bool GSYM, typename Pre>
+template <typename T, bool GSYM, typename G, typename Pre>
 __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPlan<T>& plan, const FTile& ft,
-                                            T (*s)[kEdgePts], int ltid, T& un, T& vn, int& ci, int& cj, int& cq,
+                                            T (*s)[G::pts], int ltid, T& un, T& vn, int& ci, int& cj, int& cq,
                                             Pre&& pre) {
   using RP = RegionPlanes<GSYM>;
   const int nx = a.g.nx, N = a.g.N;
@@ -2059,8 +2074,8 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
     cin[3] = cst(P.af, fo + oc, SP_AF, 0, 0);
     cin[4] = ldg(a.up, fo + oc); cin[5] = ldg(a.vp, fo + oc);
   }
-  // the region (at most kEdgePts <= 2 kEdgeGroup points): each thread loads
-  // points ltid and ltid + kEdgeGroup, every load issued before the first
+  // the region (at most G::pts <= 2 G::group points): each thread loads
+  // points ltid and ltid + G::group, every load issued before the first
   // LDS write
   struct Pt {
     T u, v, g00, g01, g10, g11, sina, h, hn;
@@ -2068,12 +2083,12 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
   // the stencils read the tile's rows across the region and its columns
   // across the region, not the region's four 3 x 3 corner blocks
   auto read = [&](int e) {
-    const int rr = rcols == 9 ? e / 9 : rcols == kEdgeTile + 6 ? e / (kEdgeTile + 6) : e / rcols;
+    const int rr = rcols == 9 ? e / 9 : rcols == G::tile + 6 ? e / (G::tile + 6) : e / rcols;
     const int cc = e - rr * rcols;
     return (rr >= 3 && rr < 3 + crows) || (cc >= 3 && cc < 3 + ccols);
   };
   auto fetch = [&](int e, Pt& p) {
-    const int rr = rcols == 9 ? e / 9 : rcols == kEdgeTile + 6 ? e / (kEdgeTile + 6) : e / rcols;
+    const int rr = rcols == 9 ? e / 9 : rcols == G::tile + 6 ? e / (G::tile + 6) : e / rcols;
     const int cc = e - rr * rcols;
     const uint32_t o = fo + static_cast<uint32_t>(ra + rr) * NS + static_cast<uint32_t>(ca + cc);
     p.u = ldg(a.u, o);
@@ -2120,7 +2135,7 @@ __device__ __forceinline__ bool frame_cell2(const MarchArgs<T>& a, const HaloPla
     s[RP::cu][e] = contra(p.g00, p.g01, p.u, p.v, p.sina);
     s[RP::cv][e] = GSYM ? contra(p.g01, p.g00, p.u, p.v, p.sina) : contra(p.g10, p.g11, p.u, p.v, p.sina);
   };
-  const int e1 = ltid + kEdgeGroup;
+  const int e1 = ltid + G::group;
   const bool r0 = ltid < npts && read(ltid), r1 = e1 < npts && read(e1);
   Pt pt0, pt1;
   if (r0) fetch(ltid, pt0);
@@ -2262,10 +2277,10 @@ __global__ __launch_bounds__(256) void k_frame(MarchArgs<T> a, HaloArgs<T> hh, i
 // Cube-edge frame kernel: the frame cells, both halo fills and the next
 // stage's edge pass in one launch.
 //
-// Block b takes cube edge e = b / nseg and the kSeg along-edge positions
+// Block b takes cube edge e = b / nseg and the G::seg along-edge positions
 // [p0, p1) of its first face edge (A = (t, d)); the second face edge (B,
 // across the joint) owns the mirrored positions.  Each half of the block
-// (kEdgeGroup threads, two per cell: frame_cell2) evaluates the frame cells
+// (G::group threads, two per cell: frame_cell2) evaluates the frame cells
 // of one side over its owned positions widened by 3 on either end and keeps
 // their new velocity in LDS.  A halo cell at an owned position takes its two
 // interpolation sources from the other side's frame cells, at most 3
@@ -2322,21 +2337,21 @@ __device__ __forceinline__ int eside_point(const ESide& S, int i, int j) {
   return (i - (S.ft.cr0 - 3)) * (S.ft.ccols + 6) + (j - (S.ft.cc0 - 3));
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, typename G>
 struct FrameEdgeLds {
-  T s[2][RegionPlanes<GSYM>::n][kEdgePts];  // the two sides' regions (frame_cell2's planes)
-  T ur[2][3 * kEdgeTile];  // new velocity of the evaluated frame cells
-  T vr[2][3 * kEdgeTile];
-  T uh[2][3][kSeg];        // halo values at the owned positions, layers 1..3
-  T vh[2][3][kSeg];
-  T rot[kPlanRot][2 * kHaloLanes];  // rotation coefficients of the halo entries
+  T s[2][RegionPlanes<GSYM>::n][G::pts];  // the two sides' regions (frame_cell2's planes)
+  T ur[2][3 * G::tile];  // new velocity of the evaluated frame cells
+  T vr[2][3 * G::tile];
+  T uh[2][3][G::seg];    // halo values at the owned positions, layers 1..3
+  T vh[2][3][G::seg];
+  T rot[kPlanRot][2 * G::halo_lanes];  // rotation coefficients of the halo entries
 };
 
 // Contravariant normal velocity (edge_body's stencil value) and new height of
 // point (i, j) of side X: halo points take the block's fill values, interior
 // points its frame values.
-template <typename T, bool GSYM>
-__device__ __forceinline__ void eside_sample(const FrameEdgeLds<T, GSYM>& L, int X, const ESide& S, int N, int i,
+template <typename T, bool GSYM, typename G>
+__device__ __forceinline__ void eside_sample(const FrameEdgeLds<T, GSYM, G>& L, int X, const ESide& S, int N, int i,
                                              int j, int pos, T& w, T& h) {
   using RP = RegionPlanes<GSYM>;
   const bool ns = S.d == DS || S.d == DN;
@@ -2361,15 +2376,15 @@ __device__ __forceinline__ void eside_sample(const FrameEdgeLds<T, GSYM>& L, int
 // Edge velocity (and the six heights) of side X at position pos: the six
 // stencil points across the edge line (remap.py:42-55 at rows / columns 3,
 // N-3), as edge_body evaluates them.
-template <typename T, bool GSYM>
-__device__ __forceinline__ T eside_vel(const FrameEdgeLds<T, GSYM>& L, int X, const ESide& S, int N, int pos, T* hv) {
+template <typename T, bool GSYM, typename G>
+__device__ __forceinline__ T eside_vel(const FrameEdgeLds<T, GSYM, G>& L, int X, const ESide& S, int N, int pos, T* hv) {
   const bool ns = S.d == DS || S.d == DN;
   const int line = (S.d == DS || S.d == DW) ? HALO : N - HALO;
   T w[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const int i = ns ? line - 3 + k : HALO + pos, j = ns ? HALO + pos : line - 3 + k;
-    eside_sample<T, GSYM>(L, X, S, N, i, j, pos, w[k], hv[k]);
+    eside_sample<T, GSYM, G>(L, X, S, N, i, j, pos, w[k], hv[k]);
   }
   return T(0.5) * (e5(w[0], w[1], w[2], w[3], w[4]) + w5(w[1], w[2], w[3], w[4], w[5]));
 }
@@ -2395,14 +2410,14 @@ __device__ __forceinline__ T eside_dz(const EdgeArgs<T>& en, const ESide& S, int
   return ns ? dzx(en.P, oc) : dzy(en.P, oc);
 }
 
-template <typename T, bool GSYM>
+template <typename T, bool GSYM, typename G>
 __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const EdgeArgs<T>& en, int blk,
-                                                FrameEdgeLds<T, GSYM>& L) {
+                                                FrameEdgeLds<T, GSYM, G>& L) {
   using RP = RegionPlanes<GSYM>;
   const int nx = a.g.nx, N = a.g.N;
   const int tid = static_cast<int>(threadIdx.x);
-  const int nseg = (nx + kSeg - 1) / kSeg;
-  const int e = blk / nseg, p0 = (blk - e * nseg) * kSeg, p1 = min(p0 + kSeg, nx);
+  const int nseg = (nx + G::seg - 1) / G::seg;
+  const int e = blk / nseg, p0 = (blk - e * nseg) * G::seg, p1 = min(p0 + G::seg, nx);
   const int ta = en.ue[e] >> 2, da = en.ue[e] & 3;
   const int tb = en.R.r[ta][da][0], db = en.R.r[ta][da][1], flip = en.R.r[ta][da][2];
   const ESide A = eside(nx, N, ta, da, p0, p1);
@@ -2411,14 +2426,14 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
   const uint32_t NS = static_cast<uint32_t>(N), NN = NS * NS;
 
   // the halo entries of the owned positions and of the corners they carry
-  // (threads 0 .. 2 kHaloLanes - 1, side tid / kHaloLanes): their plan data
+  // (threads 0 .. 2 G::halo_lanes - 1, side tid / G::halo_lanes): their plan data
   // is fetched with the frame loads, the rotation coefficients staged in LDS
-  const int hs = tid >= kHaloLanes ? 1 : 0, hk = tid - hs * kHaloLanes;
+  const int hs = tid >= G::halo_lanes ? 1 : 0, hk = tid - hs * G::halo_lanes;
   const ESide H = hs ? B : A;
   const int nown = H.hi - H.lo;
   const bool we = H.d == DW || H.d == DE;
   const int c_lo = (we && H.lo == 0) ? 1 : 0, c_hi = (we && H.hi == nx) ? 1 : 0;
-  const bool hact = tid < 2 * kHaloLanes && hk < 3 * nown + 9 * (c_lo + c_hi);
+  const bool hact = tid < 2 * G::halo_lanes && hk < 3 * nown + 9 * (c_lo + c_hi);
   int hq = 0, hlayer = 0, hpos = 0, hrow = 0, hcol = 0;
   if (hact) {
     if (hk < 3 * nown) {
@@ -2455,11 +2470,11 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
 
   // frame cells of both sides
   {
-    const int side = tid >= kEdgeGroup ? 1 : 0;
+    const int side = tid >= G::group ? 1 : 0;
     const ESide S = side ? B : A;
     T un, vn;
     int ci, cj, cq;
-    if (frame_cell2<T, GSYM>(a, plan, S.ft, L.s[side], tid - side * kEdgeGroup, un, vn, ci, cj, cq, stage_rot)) {
+    if (frame_cell2<T, GSYM, G>(a, plan, S.ft, L.s[side], tid - side * G::group, un, vn, ci, cj, cq, stage_rot)) {
       L.ur[side][cq] = un;
       L.vr[side][cq] = vn;
       const bool ns = S.d == DS || S.d == DN;
@@ -2519,7 +2534,7 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
   // lanes hold the same bits) halved
   if (eact) {
     T hv[6];
-    const T vel = eside_vel<T, GSYM>(L, ex, E, N, epos, hv);
+    const T vel = eside_vel<T, GSYM, G>(L, ex, E, N, epos, hv);
     const T avg = (vel + pair_swap(vel)) / T(2.0);
     const int o = (E.t * 4 + E.d) * nx + epos;
     en.evel[o] = avg;
@@ -2531,9 +2546,9 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
 }
 
 template <typename T, bool GSYM>
-__global__ __launch_bounds__(kFrameEdgeThreads, 3) void k_frame_edge(MarchArgs<T> a, EdgeArgs<T> en) {
-  __shared__ FrameEdgeLds<T, GSYM> L;
-  frame_edge_body<T, GSYM>(a, en, static_cast<int>(blockIdx.x), L);
+__global__ __launch_bounds__(EdgeGeo<kSeg>::threads, 3) void k_frame_edge(MarchArgs<T> a, EdgeArgs<T> en) {
+  __shared__ FrameEdgeLds<T, GSYM, EdgeGeo<kSeg>> L;
+  frame_edge_body<T, GSYM, EdgeGeo<kSeg>>(a, en, static_cast<int>(blockIdx.x), L);
 }
 
 // Kernel arguments of one RK stage of one context.
@@ -2647,12 +2662,13 @@ __global__ __launch_bounds__(256) void k_halo_uv_edge_batch(BatchLaunch<T> b) {
 
 // Each member's cube-edge frame launch (as k_frame_edge).
 template <typename T, bool GSYM>
-__global__ __launch_bounds__(kFrameEdgeThreads, 3) void k_frame_edge_batch(BatchLaunch<T> b) {
-  __shared__ FrameEdgeLds<T, GSYM> L;
+__global__ __launch_bounds__(EdgeGeo<batch_seg<T>()>::threads, 3) void k_frame_edge_batch(BatchLaunch<T> b) {
+  using G = EdgeGeo<batch_seg<T>()>;
+  __shared__ FrameEdgeLds<T, GSYM, G> L;
   const int blk = static_cast<int>(blockIdx.x);
   const int m = b.blk2m[blk];
   const StageArgs<T>& e = batch_entry(b, m);
-  frame_edge_body<T, GSYM>(e.mf, e.en, blk - b.first[m], L);
+  frame_edge_body<T, GSYM, G>(e.mf, e.en, blk - b.first[m], L);
 }
 
 // ---------------------------------------------------------------------------
@@ -3380,8 +3396,9 @@ int run_stage(const sw_ctx* c, const T* h, const T* hp, const T* u, const T* v, 
       // frame cells, velocity and height halo fill and the next edge pass in
       // one launch
       const dim3 ge(12 * ((c->nx + kSeg - 1) / kSeg));
-      return c->gsym ? launch_n(c, K_FRAME, k_frame_edge<T, true>, ge, kFrameEdgeThreads, A.mf, A.en)
-                     : launch_n(c, K_FRAME, k_frame_edge<T, false>, ge, kFrameEdgeThreads, A.mf, A.en);
+      constexpr int nt = EdgeGeo<kSeg>::threads;
+      return c->gsym ? launch_n(c, K_FRAME, k_frame_edge<T, true>, ge, nt, A.mf, A.en)
+                     : launch_n(c, K_FRAME, k_frame_edge<T, false>, ge, nt, A.mf, A.en);
     }
     // frame cells and the new height's halo fill in one launch
     rc = c->gsym ? launch(c, K_FRAME, k_frame<T, true>, gr, A.mf, A.hh, nft)
@@ -4032,7 +4049,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
                                     6 * frame_tiles(c->nx) + static_cast<int>(blocks_for(c->plan_n)),
                                     static_cast<int>(blocks_for(c->plan_n) +
                                                      blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))),
-                                    12 * ((c->nx + kSeg - 1) / kSeg)};
+                                    12 * ((c->nx + batch_seg<T>() - 1) / batch_seg<T>())};
     for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
   }
   // byte layout of the device copy
@@ -4128,10 +4145,11 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
         if (fedge) {
           // frame cells, both halo fills and the next edge pass per cube edge
           const dim3 ge(first[BK_FEDGE][active]);
+          constexpr int nt = EdgeGeo<batch_seg<T>()>::threads;
           if (gsym)
-            hipLaunchKernelGGL((k_frame_edge_batch<T, true>), ge, dim3(kFrameEdgeThreads), 0, s, L[BK_FEDGE]);
+            hipLaunchKernelGGL((k_frame_edge_batch<T, true>), ge, dim3(nt), 0, s, L[BK_FEDGE]);
           else
-            hipLaunchKernelGGL((k_frame_edge_batch<T, false>), ge, dim3(kFrameEdgeThreads), 0, s, L[BK_FEDGE]);
+            hipLaunchKernelGGL((k_frame_edge_batch<T, false>), ge, dim3(nt), 0, s, L[BK_FEDGE]);
           SW_HIP(hipGetLastError());
           continue;
         }

@@ -1,7 +1,8 @@
 """The cube-edge frame kernel's block decomposition, modelled on the CPU.
 
 k_frame_edge (csrc/barotr.hip: frame_edge_body, eside) gives block b the
-cube edge b / nseg and 42 along-edge positions (kSeg) of its first face edge; the
+cube edge b / nseg and seg along-edge positions of its first face edge (42 in
+single-context launches, 42 / 28 in batched FP64 / FP32 ones); the
 face edge across the joint owns the mirrored positions.  Each side evaluates
 its frame cells over the owned positions widened by 3 (its tile), fills the
 velocity and height halo of the owned positions (and, at positions 0 / nx-1
@@ -28,7 +29,7 @@ from licom4py_gpu_amd import _hip, duogrid
 
 DE, DW, DN, DS = 0, 1, 2, 3
 HALO = 3
-SEG = 42  # owned positions per block (kSeg)
+SEGS = (42, 28)  # owned positions per block: kSeg, batch_seg<float>()
 
 
 def _halo_entry(nx, N, t, row, col):
@@ -105,25 +106,26 @@ def _side(nx, N, t, d, lo, hi):
     return {"t": t, "d": d, "lo": lo, "hi": hi, "cells": cells, "tile": tile}
 
 
-@pytest.mark.parametrize("nx", [7, 8, 15, 16, 22, 48, 96, 192, 384, 768, 1536])
-def test_frame_edge_blocks_cover_every_cell_once(nx):
+@pytest.mark.parametrize("nx,seg", [(nx, SEGS[0]) for nx in (7, 8, 15, 16, 22, 48, 96, 192, 384, 768, 1536)]
+                         + [(nx, SEGS[1]) for nx in (7, 8, 15, 16, 22, 29, 48, 96, 192, 384, 768)])
+def test_frame_edge_blocks_cover_every_cell_once(nx, seg):
     N = nx + 6
     R = _hip.routing_table()
     plan = _plan(nx, duogrid.grid(nx), R)
     ue = [t * 4 + d for t in range(6) for d in range(4) if t * 4 + d < R[t][d][0] * 4 + R[t][d][1]]
     assert len(ue) == 12
-    nseg = (nx + SEG - 1) // SEG
+    nseg = (nx + seg - 1) // seg
     frame, halo, edge = {}, {}, {}
     for blk in range(12 * nseg):
-        e, p0 = blk // nseg, (blk % nseg) * SEG
-        p1 = min(p0 + SEG, nx)
+        e, p0 = blk // nseg, (blk % nseg) * seg
+        p1 = min(p0 + seg, nx)
         ta, da = ue[e] >> 2, ue[e] & 3
         tb, db, flip = (int(x) for x in R[ta][da])
         S = (_side(nx, N, ta, da, p0, p1),
              _side(nx, N, tb, db, nx - p1 if flip else p0, nx - p0 if flip else p1))
         for X in S:
             cr0, crows, cc0, ccols = X["tile"]
-            assert crows * ccols <= 3 * (SEG + 6) and (crows + 6) * (ccols + 6) <= (SEG + 12) * 9  # LDS sizes
+            assert crows * ccols <= 3 * (seg + 6) and (crows + 6) * (ccols + 6) <= (seg + 12) * 9  # LDS sizes
             ns = X["d"] in (DS, DN)
             for (t, i, j) in X["cells"]:
                 pos = j - 3 if ns else i - 3
@@ -135,7 +137,7 @@ def test_frame_edge_blocks_cover_every_cell_once(nx):
             nown = H["hi"] - H["lo"]
             we = H["d"] in (DW, DE)
             c_lo, c_hi = int(we and H["lo"] == 0), int(we and H["hi"] == nx)
-            assert 3 * nown + 9 * (c_lo + c_hi) <= (3 * SEG + 18 + 63) // 64 * 64  # halo lanes per side
+            assert 3 * nown + 9 * (c_lo + c_hi) <= (3 * seg + 18 + 63) // 64 * 64  # halo lanes per side
             for k in range(3 * nown + 9 * (c_lo + c_hi)):
                 if k < 3 * nown:
                     layer, pos = k // nown + 1, H["lo"] + k % nown
