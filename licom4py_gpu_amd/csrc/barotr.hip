@@ -1997,6 +1997,7 @@ struct EdgeGeo {
   static constexpr int pts = (tile + 6) * 9;
   static constexpr int group = (6 * tile + 63) / 64 * 64;
   static constexpr int halo_lanes = (3 * SEG + 18 + 63) / 64 * 64;
+  static constexpr int halo_max = 3 * SEG + 18;  // halo entries per side
   static constexpr int threads = 2 * group;
   static_assert(pts <= 2 * group, "two region points per thread");
   static_assert(2 * halo_lanes <= threads, "halo entries fit the block");
@@ -2004,15 +2005,12 @@ struct EdgeGeo {
 };
 // Segment lengths: single-context launches use 42 positions (228 blocks at
 // C768, one round on the device, where the block's latency sets the time).
-// Batched launches run many rounds, where blocks resident per CU set the
-// throughput: FP32 blocks of 28 positions (40 KB LDS, 512 threads) fit three
-// per CU where those of 42 fit two; FP64 ones of 42 (115 KB) stay one per CU,
-// and shorter FP64 segments measured slower (the margins' loads).
+// Batched launches run many rounds, where the blocks resident per CU set the
+// throughput: blocks of 28 positions (512 threads; LDS 77 KB FP64, 39 KB
+// FP32) fit two (FP64) or three (FP32) per CU, where blocks of 42 (106 KB /
+// 53 KB) fit one or two.
 constexpr int kSeg = 42;
-template <typename T>
-constexpr int batch_seg() {
-  return sizeof(T) == 8 ? 42 : 28;
-}
+constexpr int kBatchSeg = 28;
 
 // Planes of frame_cell2's LDS region: u v g00 g01 [g10 g11] sina h hn (the
 // a_gct planes g10, g11 only when the metric is not symmetric).
@@ -2344,7 +2342,7 @@ struct FrameEdgeLds {
   T vr[2][3 * G::tile];
   T uh[2][3][G::seg];    // halo values at the owned positions, layers 1..3
   T vh[2][3][G::seg];
-  T rot[kPlanRot][2 * G::halo_lanes];  // rotation coefficients of the halo entries
+  T rot[kPlanRot][2 * G::halo_max];  // rotation coefficients of the halo entries, side by side
 };
 
 // Contravariant normal velocity (edge_body's stencil value) and new height of
@@ -2464,7 +2462,7 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
   auto stage_rot = [&]() {
     if (hact) {
 #pragma unroll
-      for (int k = 0; k < kPlanRot; ++k) L.rot[k][tid] = R[k];
+      for (int k = 0; k < kPlanRot; ++k) L.rot[k][hs * G::halo_max + hk] = R[k];
     }
   };
 
@@ -2514,7 +2512,7 @@ __device__ __forceinline__ void frame_edge_body(const MarchArgs<T>& a, const Edg
     src(hs1, q1, e1);
     T Rl[kPlanRot];
 #pragma unroll
-    for (int k = 0; k < kPlanRot; ++k) Rl[k] = L.rot[k][tid];
+    for (int k = 0; k < kPlanRot; ++k) Rl[k] = L.rot[k][hs * G::halo_max + hk];
     T uo, vo;
     halo_uv_combine(Rl, hc0, hc1, L.ur[y][q0], L.vr[y][q0], L.ur[y][q1], L.vr[y][q1], uo, vo);
     const T hh = hlayer ? L.s[hs][RP::hn][eside_point(H, hrow, hcol)]
@@ -2662,8 +2660,8 @@ __global__ __launch_bounds__(256) void k_halo_uv_edge_batch(BatchLaunch<T> b) {
 
 // Each member's cube-edge frame launch (as k_frame_edge).
 template <typename T, bool GSYM>
-__global__ __launch_bounds__(EdgeGeo<batch_seg<T>()>::threads, 3) void k_frame_edge_batch(BatchLaunch<T> b) {
-  using G = EdgeGeo<batch_seg<T>()>;
+__global__ __launch_bounds__(EdgeGeo<kBatchSeg>::threads, 3) void k_frame_edge_batch(BatchLaunch<T> b) {
+  using G = EdgeGeo<kBatchSeg>;
   __shared__ FrameEdgeLds<T, GSYM, G> L;
   const int blk = static_cast<int>(blockIdx.x);
   const int m = b.blk2m[blk];
@@ -4049,7 +4047,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
                                     6 * frame_tiles(c->nx) + static_cast<int>(blocks_for(c->plan_n)),
                                     static_cast<int>(blocks_for(c->plan_n) +
                                                      blocks_for(24 * kEdgeLanes * static_cast<size_t>(c->nx))),
-                                    12 * ((c->nx + batch_seg<T>() - 1) / batch_seg<T>())};
+                                    12 * ((c->nx + kBatchSeg - 1) / kBatchSeg)};
     for (int k = 0; k < BK_CLASSES; ++k) first[k][q + 1] = first[k][q] + blocks[k];
   }
   // byte layout of the device copy
@@ -4145,7 +4143,7 @@ int run_batch(sw_batch* b, const sw_member* mem, int count, int rk, int flags, h
         if (fedge) {
           // frame cells, both halo fills and the next edge pass per cube edge
           const dim3 ge(first[BK_FEDGE][active]);
-          constexpr int nt = EdgeGeo<batch_seg<T>()>::threads;
+          constexpr int nt = EdgeGeo<kBatchSeg>::threads;
           if (gsym)
             hipLaunchKernelGGL((k_frame_edge_batch<T, true>), ge, dim3(nt), 0, s, L[BK_FEDGE]);
           else

@@ -2,7 +2,7 @@
 
 k_frame_edge (csrc/barotr.hip: frame_edge_body, eside) gives block b the
 cube edge b / nseg and seg along-edge positions of its first face edge (42 in
-single-context launches, 42 / 28 in batched FP64 / FP32 ones); the
+single-context launches, 28 in batched ones); the
 face edge across the joint owns the mirrored positions.  Each side evaluates
 its frame cells over the owned positions widened by 3 (its tile), fills the
 velocity and height halo of the owned positions (and, at positions 0 / nx-1
@@ -29,7 +29,7 @@ from licom4py_gpu_amd import _hip, duogrid
 
 DE, DW, DN, DS = 0, 1, 2, 3
 HALO = 3
-SEGS = (42, 28)  # owned positions per block: kSeg, batch_seg<float>()
+SEGS = (42, 28)  # owned positions per block: kSeg, kBatchSeg
 
 
 def _halo_entry(nx, N, t, row, col):
@@ -137,7 +137,7 @@ def test_frame_edge_blocks_cover_every_cell_once(nx, seg):
             nown = H["hi"] - H["lo"]
             we = H["d"] in (DW, DE)
             c_lo, c_hi = int(we and H["lo"] == 0), int(we and H["hi"] == nx)
-            assert 3 * nown + 9 * (c_lo + c_hi) <= (3 * seg + 18 + 63) // 64 * 64  # halo lanes per side
+            assert 3 * nown + 9 * (c_lo + c_hi) <= 3 * seg + 18  # halo entries per side (LDS rotation slots)
             for k in range(3 * nown + 9 * (c_lo + c_hi)):
                 if k < 3 * nown:
                     layer, pos = k // nown + 1, H["lo"] + k % nown
