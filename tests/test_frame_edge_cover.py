@@ -107,7 +107,7 @@ def _side(nx, N, t, d, lo, hi):
 
 
 @pytest.mark.parametrize("nx,seg", [(nx, SEGS[0]) for nx in (7, 8, 15, 16, 22, 48, 96, 192, 384, 768, 1536)]
-                         + [(nx, SEGS[1]) for nx in (7, 8, 15, 16, 22, 29, 48, 96, 192, 384, 768)])
+                         + [(nx, SEGS[1]) for nx in (7, 8, 15, 16, 22, 29, 48, 96, 192, 384, 768, 1536)])
 def test_frame_edge_blocks_cover_every_cell_once(nx, seg):
     N = nx + 6
     R = _hip.routing_table()
