@@ -54,3 +54,19 @@ def max_ulp_diff(a, b) -> int:
     a = np.where(a < 0, np.int64(-0x8000000000000000) - a, a)
     b = np.where(b < 0, np.int64(-0x8000000000000000) - b, b)
     return int(np.max(np.abs(a - b))) if a.size else 0
+
+
+def generic_consts(consts, which):
+    """Copies of the consts with layout properties the kernels exploit broken:
+    "dz" (dzph not uniform), "shared" (rdx differs on face 2), "gsym" (a_gct
+    not symmetric on face 3)."""
+    c = [np.array(x, copy=True) for x in consts]
+    if "dz" in which:
+        r = np.random.default_rng(5)
+        c[0] = c[0] + r.uniform(-20, 20, size=c[0].shape)
+        c[1] = c[1] + r.uniform(-20, 20, size=c[1].shape)
+    if "shared" in which:
+        c[2][2] *= 1.0 + 1e-3
+    if "gsym" in which:
+        c[12][3, ..., 0, 1] *= 1.0 + 1e-3
+    return tuple(c)

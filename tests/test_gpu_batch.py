@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import barotr_np as B
-from tests._util import grid_consts, random_core
+from tests._util import generic_consts, grid_consts, random_core
 
 pytestmark = pytest.mark.gpu
 
@@ -207,3 +207,34 @@ def test_batch_frame_forms_match_alone_and_oracle(graph):
         want = B.barotr_rk2_core(tuple(st), m.tag["consts"], m.nbb, m.dtb)
         for i, idx in enumerate((0, 2, 3)):
             assert np.array_equal(m.out[i].cpu().numpy(), want[idx]), (mixed, idx)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_generic_layout_matches_alone(dtype):
+    """Members whose consts lack the grid's uniform / face-shared / symmetric
+    structure run the kernels' general paths (k_frame_edge_batch<T, false>
+    among them): each equals its run alone, and FP64 members the oracle."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    from licom4py_gpu_amd.barotr import SWContext
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    members = []
+    for k, (nx, nbb, dtb) in enumerate([(8, 3, 40.0), (13, 2, 25.0), (16, 3, 30.0)]):
+        _, base = grid_consts(nx)
+        consts = generic_consts(base, ("dz", "shared", "gsym"))
+        ctx = SWContext(consts, tdt)
+        f = ctx.layout_flags()
+        assert not (f["dz_uniform"] or f["shared_metrics"] or f["gct_symmetric"])
+        assert ctx.frame_form()
+        core_np = random_core(nx, seed=20 + k, distinct_p=False)
+        core = [torch.as_tensor(x).cuda().to(tdt) for x in core_np]
+        members.append(ensemble.Member(ctx, core, nbb, dtb, tag={"consts": consts, "core": core_np}))
+    ensemble.Batch().run(members, graph=True)
+    _check_alone(torch, members)
+    if dtype == "f64":
+        for m in members:
+            core = m.tag["core"]
+            st = list(core) + [np.zeros_like(core[0])] * 17
+            want = B.barotr_rk2_core(tuple(st), m.tag["consts"], m.nbb, m.dtb)
+            for i, idx in enumerate((0, 2, 3)):
+                assert np.array_equal(m.out[i].cpu().numpy(), want[idx]), (m.ctx.nx, idx)

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import barotr_np as B
-from tests._util import grid_consts, max_ulp_diff, random_core, rel_l2, seeded_state
+from tests._util import generic_consts, grid_consts, max_ulp_diff, random_core, rel_l2, seeded_state
 
 pytestmark = pytest.mark.gpu
 
@@ -354,21 +354,6 @@ def test_rk2_c140(graph, rows_f):
         _assert_exact(f"C140 diagnostic {k}", _host(got[3 + k]), want[6 + k])
 
 
-def _generic_consts(consts, which):
-    """Copies of the consts with one layout property the kernels exploit broken."""
-    c = [np.array(x, copy=True) for x in consts]
-    nx = c[0].shape[-1] - 6
-    if "dz" in which:      # dzph not uniform -> per-cell planes
-        r = np.random.default_rng(5)
-        c[0] = c[0] + r.uniform(-20, 20, size=c[0].shape)
-        c[1] = c[1] + r.uniform(-20, 20, size=c[1].shape)
-    if "shared" in which:  # rdx differs on face 2 -> per-face metric planes
-        c[2][2] *= 1.0 + 1e-3
-    if "gsym" in which:    # a_gct not symmetric on face 3
-        c[12][3, ..., 0, 1] *= 1.0 + 1e-3
-    return tuple(c)
-
-
 @pytest.mark.parametrize("which,flags", [
     (("dz", "shared", "gsym"), (False, False, False)),
     (("shared",), (True, False, True)),
@@ -380,7 +365,7 @@ def test_generic_layout_matches_oracle(which, flags, last):
     from licom4py_gpu_amd.barotr import SWContext
     nx = 13
     _, base = grid_consts(nx)
-    consts = _generic_consts(base, which)
+    consts = generic_consts(base, which)
     ctx = SWContext(consts, torch.float64)
     f = ctx.layout_flags()
     assert (f["dz_uniform"], f["shared_metrics"], f["gct_symmetric"]) == flags
