@@ -238,3 +238,22 @@ def test_batch_generic_layout_matches_alone(dtype):
             want = B.barotr_rk2_core(tuple(st), m.tag["consts"], m.nbb, m.dtb)
             for i, idx in enumerate((0, 2, 3)):
                 assert np.array_equal(m.out[i].cpu().numpy(), want[idx]), (m.ctx.nx, idx)
+
+
+def test_batch_ragged_frame_segments_match_alone_and_oracle():
+    """Resolutions whose cube edges leave a one-position last segment of the
+    batched frame launch (C29, C57: 28 + 1, 28 + 28 + 1) and the smallest one
+    (C7): each member equals its run alone and the oracle."""
+    torch = _torch()
+    from licom4py_gpu_amd import ensemble
+    members = _members(torch, torch.float64, [(29, 2, 25.0), (57, 2, 20.0)], seed=31)
+    assert all(m.ctx.frame_form() for m in members)  # the batch runs k_frame_edge_batch
+    members += _members(torch, torch.float64, [(7, 3, 30.0)], seed=41)
+    ensemble.Batch().run(members, graph=True)
+    _check_alone(torch, members)
+    for m in members:
+        core = m.tag["core"]
+        st = list(core) + [np.zeros_like(core[0])] * 17
+        want = B.barotr_rk2_core(tuple(st), m.tag["consts"], m.nbb, m.dtb)
+        for i, idx in enumerate((0, 2, 3)):
+            assert np.array_equal(m.out[i].cpu().numpy(), want[idx]), (m.ctx.nx, idx)
