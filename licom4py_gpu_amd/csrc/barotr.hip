@@ -2659,8 +2659,18 @@ __global__ __launch_bounds__(256) void k_halo_uv_edge_batch(BatchLaunch<T> b) {
 }
 
 // Each member's cube-edge frame launch (as k_frame_edge).
+// Waves per SIMD of the batched frame launch: its blocks per CU (LDS-limited:
+// two FP64 ones, one with the four a_gct planes; three FP32 ones, two with
+// them) times their 8 waves, over 4 SIMDs; the bound keeps the registers
+// within that occupancy.
 template <typename T, bool GSYM>
-__global__ __launch_bounds__(EdgeGeo<kBatchSeg>::threads, 3) void k_frame_edge_batch(BatchLaunch<T> b) {
+constexpr int frame_batch_waves() {
+  return sizeof(T) == 8 ? (GSYM ? 4 : 2) : (GSYM ? 6 : 4);
+}
+
+template <typename T, bool GSYM>
+__global__ __launch_bounds__(EdgeGeo<kBatchSeg>::threads, (frame_batch_waves<T, GSYM>()))
+void k_frame_edge_batch(BatchLaunch<T> b) {
   using G = EdgeGeo<kBatchSeg>;
   __shared__ FrameEdgeLds<T, GSYM, G> L;
   const int blk = static_cast<int>(blockIdx.x);
