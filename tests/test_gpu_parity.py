@@ -151,6 +151,39 @@ def test_rk3_matches_oracle():
         _assert_exact(f"rk3 {B.STATE_KEYS[k]}", _host(gv), wv)
 
 
+def test_create_rejects_invalid_consts():
+    """sw_create validates what its kernels index with: the interior / halo
+    masks and the local halo indices of the strips (a RuntimeError naming the
+    table); shapes and the dtype are checked on the Python side."""
+    torch = _torch()
+    from licom4py_gpu_amd.barotr import SWContext
+    _, consts = grid_consts(8)
+
+    def broken(k, edit):
+        c = [np.array(x, copy=True) for x in consts]
+        edit(c[k])
+        return tuple(c)
+
+    def set_(idx, val):
+        def f(a):
+            a[idx] = val
+        return f
+
+    with pytest.raises(RuntimeError, match="inner/outer"):
+        SWContext(broken(10, set_((0, 4, 4), 0.0)), torch.float64)   # an interior cell marked halo
+    with pytest.raises(RuntimeError, match="loc_i_local"):
+        SWContext(broken(6, set_((0, 5, 0), 0)), torch.float64)      # W strip index out of range
+    with pytest.raises(RuntimeError, match="loc_j_local"):
+        SWContext(broken(7, set_((2, 0, 5), 99)), torch.float64)     # S strip index out of range
+    c = list(consts)
+    c[3] = np.array(c[3][:, :-1, :-1], copy=True)
+    with pytest.raises(ValueError, match="shape"):
+        SWContext(tuple(c), torch.float64)
+    with pytest.raises(ValueError, match="dtype"):
+        SWContext(consts, torch.float16)
+    SWContext(consts, torch.float64)  # the unbroken tables still build a context
+
+
 def test_distinct_previous_fields():
     """h0p != h0 at entry (allowed by the signature): stage 1 must use h0p."""
     torch = _torch()
