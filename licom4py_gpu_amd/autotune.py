@@ -25,8 +25,21 @@ from typing import Dict, Iterable, Optional, Sequence, Tuple
 
 CANDIDATES: Tuple[int, ...] = (8, 12, 16, 20, 24, 26, 28, 30, 32, 36, 40, 48)
 CACHE_ENV = "LICOM_SW_TUNE_CACHE"
-_KERNEL_SET = "fused3"  # cache entries of other kernel generations are ignored
+_KERNEL_SET = "fused4"  # cache entries of other kernel generations are ignored
 SPLIT_SHARES = {"float64": (0.62, 0.66, 0.69, 0.72), "float32": (0.55, 0.6, 0.65)}
+
+
+def one_round_rows(nx: int, fp64: bool, cus: int) -> int:
+    """The lowest band height whose fused-kernel waves (6 faces x ceil(nx / 52)
+    strips x bands) all fit one round of resident waves (8 per CU in FP64, 12
+    in FP32), when it exceeds the fixed candidates (64 rows); else 0.  At C1536
+    it is 140 rows FP64 and 91 FP32, the fastest heights measured there."""
+    strips = -(-nx // 52)
+    bands = (8 if fp64 else 12) * cus // (6 * strips)
+    if bands < 1:
+        return 0
+    rows = -(-nx // bands)
+    return rows if rows > 64 else 0
 
 
 def _cache_path() -> str:
@@ -72,6 +85,10 @@ def tune(ctx, core6: Sequence, dt: float, substeps: int = 2, reps: int = 2,
         return dict(best, cached=True)
     ctx.set_band_split(0)  # uniform bands while the heights are swept
     cands = [c for c in candidates if c <= max(4, ctx.nx)]
+    cus = torch.cuda.get_device_properties(ctx.device).multi_processor_count
+    r1 = one_round_rows(ctx.nx, ctx.dtype == torch.float64, cus)
+    if r1 and r1 not in cands:
+        cands.append(r1)
     times = {}
 
     def per_launch(prof, *names):

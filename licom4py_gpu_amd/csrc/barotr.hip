@@ -4295,9 +4295,14 @@ extern "C" {
 int default_rows_f(int nx, int cus, int dtype) {
   const long S = (nx + kFStride - 1) / kFStride;
   const long slots = (dtype == SW_F64 ? 8L : 12L) * (cus > 0 ? cus : 256);
+  // beyond the list, the lowest height whose waves all fit one round (C1536:
+  // 140 rows FP64, 91 FP32, measured fastest there among 28 ... 200 rows)
+  const long nb1 = slots / (6 * S);
+  const int r1 = nb1 >= 1 && (nx + nb1 - 1) / nb1 > 64 ? static_cast<int>((nx + nb1 - 1) / nb1) : 0;
   int best = 32;
   long best_cost = -1;
-  for (int R : {8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64}) {
+  for (int R : {8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 64, r1}) {
+    if (R <= 0) continue;
     const long waves = 6 * S * ((nx + R - 1) / R);
     const long cost = ((waves + slots - 1) / slots) * (std::min(R, nx) + 6);
     if (best_cost < 0 || cost < best_cost) {
@@ -4313,16 +4318,21 @@ int default_rows_f(int nx, int cus, int dtype) {
 // resident blocks with more blocks than CUs, the first block on each CU
 // finishes far ahead of the others, so pairs of 2 rows_f rows are split into
 // a long band (69 % in FP64, 60 % in FP32: the measured optima at C384 and
-// C768) for the first blocks and a short one for the rest.  Returns the
-// short height (0: uniform) and sets *rows_f to the long one.
+// C768) for the first blocks and a short one for the rest.  The long bands'
+// blocks must be the older ones on their CUs (FP64: at most one per CU);
+// bands taller than 64 rows stay uniform (at C1536 the splits measured no
+// faster than the uniform one-round height).  Returns the short height (0:
+// uniform) and sets *rows_f to the long one.
 int default_rows_f2(int nx, int cus, int dtype, int* rows_f) {
   const long S = (nx + kFStride - 1) / kFStride;
   const long blocks = (6 * S * ((nx + *rows_f - 1) / *rows_f) + 3) / 4;
   const long per_cu = dtype == SW_F64 ? 2 : 3;  // resident 4-wave blocks per CU
-  if (cus <= 0 || blocks <= cus || blocks > per_cu * cus) return 0;
+  if (cus <= 0 || blocks <= cus || blocks > per_cu * cus || *rows_f > 64) return 0;
   const int span = 2 * *rows_f;
   const int lng = static_cast<int>(std::lround(span * (dtype == SW_F64 ? 0.69 : 0.60)));
   if (lng < 4 || span - lng < 4 || lng >= nx) return 0;
+  const long long_blocks = (6 * S * ((nx + span - 1) / span) + 3) / 4;
+  if (long_blocks > (per_cu - 1) * cus) return 0;
   *rows_f = lng;
   return span - lng;
 }

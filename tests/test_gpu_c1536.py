@@ -77,3 +77,21 @@ def test_c1536_fp32_against_fp64_and_mass_drift(c1536_host):
     assert rel[1] <= 1e-4 and rel[2] <= 1e-4
     assert drift[torch.float64] < 1e-14
     assert drift[torch.float32] < 1e-10
+
+
+def test_c1536_default_bands_fit_one_round(c1536_host):
+    """Without the tuner a C1536 context marches bands of the lowest height
+    whose waves fit one round of resident waves (140 rows FP64, 91 FP32 on
+    256 CUs), uniform: the fastest heights measured there."""
+    import torch
+    from licom4py_gpu_amd import barotr
+    from licom4py_gpu_amd.autotune import one_round_rows
+    host_consts, _, _ = c1536_host
+    dev = torch.device("cuda", 0)
+    dconsts = barotr.as_device_consts(host_consts, dev)
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    for dtype in (torch.float64, torch.float32):
+        ctx = barotr.SWContext(dconsts, dtype)
+        assert ctx.tuning[2] == one_round_rows(NX, dtype == torch.float64, cus) > 64, (dtype, ctx.tuning)
+        assert ctx.split == 0
+        del ctx

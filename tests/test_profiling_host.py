@@ -60,3 +60,24 @@ def test_traffic_scales_counters_by_the_copy(tmp_path):
     assert got["fetch_scale"] == pytest.approx(2.0) and got["write_scale"] == pytest.approx(1.0)
     assert got["fetch_kib_per_dispatch"] == pytest.approx(300.0)
     assert got["bytes_per_launch"] == pytest.approx((300.0 * 2 + 100.0) * 1024)
+
+
+def test_one_round_band_height():
+    """The tuner's extra candidate: the lowest band height whose fused-kernel
+    waves fit one round (8 / 12 resident waves per CU), counted by brute force
+    over heights, offered only beyond the fixed list (64 rows)."""
+    from licom4py_gpu_amd.autotune import one_round_rows
+
+    def brute(nx, fp64, cus):
+        strips = -(-nx // 52)
+        slots = (8 if fp64 else 12) * cus
+        for rows in range(1, nx + 1):
+            if 6 * strips * -(-nx // rows) <= slots:
+                return rows if rows > 64 else 0
+        return 0
+
+    assert one_round_rows(1536, True, 256) == 140 and one_round_rows(1536, False, 256) == 91
+    for nx in (7, 48, 96, 384, 768, 1000, 1536, 2048, 3072, 8192):
+        for fp64 in (True, False):
+            for cus in (80, 256, 304):
+                assert one_round_rows(nx, fp64, cus) == brute(nx, fp64, cus), (nx, fp64, cus)
